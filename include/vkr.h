@@ -1,0 +1,375 @@
+/*
+ * vkr.h - C ABI of the MI355X (gfx950) path tracer.
+ *
+ * This header is the drop-in boundary for the compute-shader path of
+ * stickyfingies/vk-renderer.  In the reference that path is a Vulkan compute
+ * pipeline:
+ *   - descriptor set 0 = { binding 0: rgba8 storage image,
+ *                          binding 1: std430 SSBO `SceneData { Triangle tris[]; }` }
+ *     (Source/GraphicsDevice.cpp:988-1010, Assets/Tracer.comp:144-149)
+ *   - a 96-byte push constant `FrameData` (Source/GraphicsDevice.cpp:1106-1111,
+ *     Assets/Tracer.comp:151-166, Include/GraphicsDevice.h:20-29)
+ *   - per frame: bind, push, vkCmdDispatch(RES/16, RES/16, 1)
+ *     (Source/GraphicsDevice.cpp:1254-1266)
+ *
+ * Every entry point below replaces one piece of that pipeline; the comment on
+ * each one names the reference code it stands in for.  All structs keep the
+ * reference's std430 byte layout (sizes and offsets are checked with
+ * static asserts at the bottom of this file).
+ *
+ * Conventions
+ *   - Every function returns a vkr_error (int8_t).  Nothing throws across the
+ *     ABI.  vkr_last_error_string() describes the last failure of the calling
+ *     thread.
+ *   - Pointers documented as "device" are HIP device pointers; "host" pointers
+ *     are read during the call only.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *   - A context is not internally synchronised: use one context per thread.
+ */
+#ifndef VKR_H
+#define VKR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Errors: values 0..3 are GraphicsDevice::Error (Include/GraphicsDevice.h:46-52). */
+/* ------------------------------------------------------------------------ */
+typedef int8_t vkr_error;
+enum {
+    VKR_SUCCESS             = 0, /* Error::SUCCESS */
+    VKR_NO_SUITABLE_GPU     = 1, /* Error::NO_SUITABLE_GPU */
+    VKR_NO_SUITABLE_SURFACE = 2, /* Error::NO_SUITABLE_SURFACE */
+    VKR_UNKNOWN             = 3, /* Error::UNKNOWN */
+    VKR_INVALID_ARGUMENT    = 4, /* build extension: bad size, null pointer, bad id */
+    VKR_OUT_OF_MEMORY       = 5, /* build extension: device or host allocation failed */
+    VKR_HIP_ERROR           = 6, /* build extension: a HIP runtime call failed */
+    VKR_UNSUPPORTED         = 7  /* build extension: feature not available for this input */
+};
+
+/* ------------------------------------------------------------------------ */
+/* std430 data layouts                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* Material types, Assets/Tracer.comp:51-52. */
+enum { VKR_MAT_DIFFUSE = 0, VKR_MAT_DIELECTRIC = 1 };
+
+/* Material, Assets/Tracer.comp:59-74 (std430: albedo@0 emissive@16
+ * roughness@28 metalness@32 type@36, size 48).  For dielectrics the index of
+ * refraction lives in `roughness` (Tracer.comp:532-539). */
+typedef struct vkr_material {
+    float    albedo[3];
+    float    _pad0;
+    float    emissive[3];
+    float    roughness;
+    float    metalness;
+    uint32_t type;
+    float    _pad1[2];
+} vkr_material;
+
+/* Triangle, Include/GraphicsDevice.h:13-18 and Assets/Tracer.comp:127-132
+ * (std430 stride 48: v0@0 v1@16 v2@32).  The 4 padding bytes after v0 may
+ * carry a material index when VKR_SCENE_TRI_MATERIAL_INDEX is set; the
+ * reference never reads them. */
+typedef struct vkr_triangle {
+    float    v0[3];
+    uint32_t mat_index;
+    float    v1[3];
+    float    _pad1;
+    float    v2[3];
+    float    _pad2;
+} vkr_triangle;
+
+/* Sphere, Assets/Tracer.comp:102-111 (mat@0 P@48 r@60, size 64). */
+typedef struct vkr_sphere {
+    vkr_material mat;
+    float        P[3];
+    float        r;
+} vkr_sphere;
+
+/* Plane dot(X, N) + len = 0, Assets/Tracer.comp:118-125 (mat@0 N@48 len@60). */
+typedef struct vkr_plane {
+    vkr_material mat;
+    float        N[3];
+    float        len;
+} vkr_plane;
+
+/* CameraData, Include/Camera.h:5-12 and Assets/Tracer.comp:22-35
+ * (four 16-byte aligned vec3: pos@0 dir@16 right@32 up@48). */
+typedef struct vkr_camera {
+    float pos[3];   float _pad0;
+    float dir[3];   float _pad1;
+    float right[3]; float _pad2;
+    float up[3];    float _pad3;
+} vkr_camera;
+
+/* FrameData push constant, Include/GraphicsDevice.h:20-29 and
+ * Assets/Tracer.comp:151-166 (aspect@0 seed@4 light_pos@16 camera@32, 96 B). */
+typedef struct vkr_frame_data {
+    float      aspect_ratio;
+    float      seed;
+    float      _pad0[2];
+    float      light_pos[3];
+    float      _pad1;
+    vkr_camera camera;
+} vkr_frame_data;
+
+/* ------------------------------------------------------------------------ */
+/* Context                                                                   */
+/* ------------------------------------------------------------------------ */
+typedef struct vkr_ctx vkr_ctx;
+
+typedef struct vkr_create_info {
+    int32_t  device; /* HIP device ordinal; -1 = the current device */
+    uint32_t flags;  /* VKR_CREATE_* */
+} vkr_create_info;
+
+enum { VKR_CREATE_DEBUG = 1u << 0 /* CreateInfo::debug, Include/GraphicsDevice.h:70-71 */ };
+
+/* Replaces GraphicsDevice::Construct's device selection and compute pipeline
+ * creation (Source/GraphicsDevice.cpp:282-343, 1089-1131).  Fails with
+ * VKR_NO_SUITABLE_GPU when no HIP device is visible or the device is not
+ * gfx950.  Allocates nothing large. */
+vkr_error vkr_create(const vkr_create_info* info, vkr_ctx** out_ctx);
+
+/* Replaces GraphicsDevice::Destruct (Source/GraphicsDevice.cpp:1138-1213).
+ * Frees every scene and buffer the context owns.  NULL is accepted. */
+vkr_error vkr_destroy(vkr_ctx* ctx);
+
+/* Replaces GraphicsDevice::WaitIdle (Source/GraphicsDevice.cpp:1344-1347). */
+vkr_error vkr_wait_idle(vkr_ctx* ctx);
+
+/* Thread-local description of the last error returned on this thread. */
+const char* vkr_last_error_string(void);
+
+/* ------------------------------------------------------------------------ */
+/* Scenes                                                                    */
+/* ------------------------------------------------------------------------ */
+enum {
+    /* Triangles take materials[tri.mat_index] instead of the single
+     * tri_material (build extension; the reference shades every triangle
+     * with `mirror`, Assets/Tracer.comp:386). */
+    VKR_SCENE_TRI_MATERIAL_INDEX = 1u << 0
+};
+
+typedef struct vkr_scene_desc {
+    const vkr_triangle* tris;        uint32_t n_tris;     /* host */
+    const vkr_sphere*   spheres;     uint32_t n_spheres;  /* host */
+    const vkr_plane*    planes;      uint32_t n_planes;   /* host */
+    const vkr_material* tri_material; /* host; NULL = the reference `mirror` (Tracer.comp:192) */
+    const vkr_material* materials;   uint32_t n_materials; /* host; per-triangle table */
+    uint32_t flags;                  /* VKR_SCENE_* */
+    uint32_t max_leaf_size;          /* BVH leaf size cap (0 = 4; at most 8) */
+    uint32_t max_depth;              /* BVH depth cap (0 = 32; at most 64) */
+} vkr_scene_desc;
+
+/* Replaces the SceneData SSBO creation and upload
+ * (Source/GraphicsDevice.cpp:796-822) and the shader-constant spheres/planes
+ * (Assets/Tracer.comp:196-211).  Builds a binned-SAH BVH over triangles and
+ * spheres on the host, uploads everything to device-local memory and returns
+ * a scene id owned by the context.  Planes stay a linear list. */
+vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* out_scene);
+
+/* Frees one scene. */
+vkr_error vkr_scene_destroy(vkr_ctx* ctx, uint32_t scene);
+
+typedef struct vkr_scene_info {
+    uint32_t n_tris, n_spheres, n_planes, n_emitters;
+    uint32_t n_nodes, n_leaves, depth, stack_needed;
+    uint64_t device_bytes;           /* nodes + primitive records + materials */
+    double   build_ms;               /* host BVH build time */
+} vkr_scene_info;
+
+vkr_error vkr_scene_get_info(vkr_ctx* ctx, uint32_t scene, vkr_scene_info* out);
+
+/* Differentiable parameters of a scene's spheres, laid out per sphere as
+ * [albedo.r, albedo.g, albedo.b, emission.r, emission.g, emission.b,
+ *  roughness] followed by one global light scale L, so
+ * n = 7 * n_spheres + 1.  The traced emissive of sphere s is L * emission_s.
+ * Writes the materials of every sphere (albedo, emissive, roughness; type and
+ * metalness stay) and recomputes the emitter list.  Host pointer.
+ * Build extension: the reference has no parameters (Tracer.comp:186-202). */
+vkr_error vkr_scene_set_sphere_params(vkr_ctx* ctx, uint32_t scene, const float* params, uint32_t n);
+
+/* ------------------------------------------------------------------------ */
+/* Tracing                                                                   */
+/* ------------------------------------------------------------------------ */
+enum {
+    VKR_KERNEL_PATH    = 0, /* Assets/Tracer.comp */
+    VKR_KERNEL_WHITTED = 1  /* Assets/Raytracer.comp (its own constant scene) */
+};
+
+enum {
+    VKR_SCHED_AUTO       = 0, /* autotuned choice, else persistent */
+    VKR_SCHED_PLAIN      = 1, /* one lane per pixel, one launch */
+    VKR_SCHED_PERSISTENT = 2  /* persistent waves refilled from a pixel pool */
+};
+
+enum {
+    /* RR / dielectric decisions are written to outputs.decisions. */
+    VKR_TRACE_RECORD_DECISIONS = 1u << 0,
+    /* RR / dielectric decisions are read from outputs.decisions instead of
+     * being drawn (the random numbers are still drawn, so the stream is
+     * unchanged).  Used for finite differences with frozen branches. */
+    VKR_TRACE_REPLAY_DECISIONS = 1u << 1
+};
+
+typedef struct vkr_trace_params {
+    uint32_t width, height;  /* image size; the reference uses RES x RES (Source/GraphicsDevice.cpp:680-681) */
+    uint32_t spp;            /* SAMPLES (Tracer.comp:180), >= 1 */
+    uint32_t max_depth;      /* DEPTH (Tracer.comp:179), 1..16 */
+    float    epsilon;        /* EPSILON (Tracer.comp:177); 0 = 1e-3 */
+    uint32_t kernel;         /* VKR_KERNEL_* */
+    uint32_t sched;          /* VKR_SCHED_* */
+    uint32_t flags;          /* VKR_TRACE_* */
+    uint32_t block;          /* threads per workgroup (64, 128, 256); 0 = tuned/default */
+    uint32_t chunk;          /* persistent pool chunk in pixels per wave grab; 0 = tuned/default */
+} vkr_trace_params;
+
+/* Caller-allocated device buffers; any may be NULL. */
+typedef struct vkr_outputs {
+    uint8_t*  rgba8;     /* W*H*4, the reference's render_target (Tracer.comp:144,592) */
+    float*    radiance;  /* W*H*4: pre-tonemap mean radiance rgb, a = 1 */
+    uint32_t* prim_id;   /* W*H: primary hit, (kind << 30) | index; kind 0 tri, 1 sphere, 2 plane; miss = 0xFFFFFFFF */
+    float*    hit_t;     /* W*H: primary hit t (t_max when missed) */
+    uint32_t* ray_count; /* W*H: trace_ray calls of the pixel (camera + bounce + shadow) */
+    uint16_t* decisions; /* W*H*spp: per-path branch bits (see VKR_TRACE_*_DECISIONS) */
+} vkr_outputs;
+
+/* Replaces Draw's compute part: vkCmdPushConstants(FrameData) +
+ * vkCmdDispatch(RES/16, RES/16, 1) (Source/GraphicsDevice.cpp:1254-1266).
+ * frame_data is used as given (the reference host overwrites aspect_ratio and
+ * seed, Source/GraphicsDevice.cpp:1258-1262; the C++ GraphicsDevice facade
+ * reproduces that).  Asynchronous on `stream`. */
+vkr_error vkr_trace_frame(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* frame_data,
+                          const vkr_trace_params* params, const vkr_outputs* outputs, void* stream);
+
+/* One job of a batched launch. */
+typedef struct vkr_scene_job {
+    uint32_t         scene;
+    vkr_frame_data   frame_data;
+    vkr_trace_params params;  /* kernel must be VKR_KERNEL_PATH; block/chunk/sched ignored */
+    vkr_outputs      outputs; /* device pointers */
+} vkr_scene_job;
+
+/* Build extension (no reference counterpart beyond Draw): traces every job in
+ * ONE persistent launch whose waves pull (job, pixel-chunk) work items from a
+ * global pool.  Results per job are identical to vkr_trace_frame. */
+vkr_error vkr_trace_batch(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n_jobs,
+                          uint32_t block, uint32_t chunk, void* stream);
+
+/* Captures a batched launch (pool reset + persistent kernel) in a hipGraph
+ * owned by the context and returns its id; vkr_graph_launch replays it. */
+vkr_error vkr_batch_graph_create(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n_jobs,
+                                 uint32_t block, uint32_t chunk, void* stream, uint32_t* out_graph);
+vkr_error vkr_graph_launch(vkr_ctx* ctx, uint32_t graph, void* stream);
+vkr_error vkr_graph_destroy(vkr_ctx* ctx, uint32_t graph);
+
+/* Build extension: path-replay adjoint of the path kernel.  For loss gradient
+ * d_radiance (device, W*H*4 floats, dLoss/d(mean radiance rgb); a ignored)
+ * accumulates dLoss/dparams into d_params (device, 7 * n_spheres + 1 floats,
+ * layout of vkr_scene_set_sphere_params; the caller zeroes it).  Paths are
+ * recomputed from the RNG stream, not stored.  Branch decisions (emitter set,
+ * Russian roulette, dielectric choice) are held fixed. */
+vkr_error vkr_adjoint(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* frame_data,
+                      const vkr_trace_params* params, const float* d_radiance,
+                      float* d_params, void* stream);
+
+/* Counters of the last trace/batch/adjoint call on this context (filled when
+ * the call's stream completes; vkr_last_counters synchronises). */
+typedef struct vkr_counters {
+    uint64_t rays;            /* trace_ray calls (camera + bounce + shadow) */
+    uint64_t shadow_rays;     /* of which next-event shadow rays */
+    uint64_t nodes_visited;   /* BVH nodes fetched (VKR_CTX_COUNT_WORK only) */
+    uint64_t tris_tested;     /* triangle tests (VKR_CTX_COUNT_WORK only) */
+    uint64_t spheres_tested;  /* sphere tests (VKR_CTX_COUNT_WORK only) */
+    uint64_t planes_tested;   /* plane tests (VKR_CTX_COUNT_WORK only) */
+    uint64_t window_overflow; /* closest-hit tie window exceeded (documented near-ties) */
+} vkr_counters;
+
+vkr_error vkr_last_counters(vkr_ctx* ctx, vkr_counters* out);
+
+enum {
+    /* Run the work-counting kernel variant (slower; fills nodes/tests). */
+    VKR_CTX_COUNT_WORK = 1u << 0
+};
+vkr_error vkr_set_ctx_flags(vkr_ctx* ctx, uint32_t flags);
+
+/* Event-timed duration in ms of the last kernel launched by this context
+ * (the events are recorded on the launch stream).  Synchronises. */
+vkr_error vkr_last_kernel_ms(vkr_ctx* ctx, float* out_ms);
+
+/* ------------------------------------------------------------------------ */
+/* Post pass and host helpers                                                */
+/* ------------------------------------------------------------------------ */
+
+/* Build extension of the present pass (Assets/Fullscreen.vert:6-11,
+ * Assets/Fullscreen.frag:14-31): temporal-variance-gated 4-tap filter of the
+ * current traced image against the previous one, bilinear with a black
+ * border, written at the swapchain size.  Device pointers. */
+vkr_error vkr_filter_frame(vkr_ctx* ctx, const uint8_t* current, const uint8_t* previous,
+                           uint32_t src_w, uint32_t src_h, uint8_t* out,
+                           uint32_t dst_w, uint32_t dst_h, void* stream);
+
+/* Camera::update (Source/Camera.cpp:91-107): orthonormal basis from pitch/yaw
+ * in degrees (pitch clamped to [-89, 89]).  pos is copied through.  Host
+ * only; needs no context or GPU. */
+vkr_error vkr_camera_from_angles(const float pos[3], float pitch, float yaw, vkr_camera* out);
+
+/* ------------------------------------------------------------------------ */
+/* Diagnostics                                                               */
+/* ------------------------------------------------------------------------ */
+/* Build extension for parity tests: evaluates one device building block of
+ * the path kernel on n records.  in: device, n * 16 floats; out: device,
+ * n * 8 floats.  Record layouts:
+ *   RAND    in  cx, cy, salt0                 out 8 successive rand() (Tracer.comp:225-234)
+ *   TRI     in  o(3) d(3) v0(3) v1(3) v2(3) eps out t (Tracer.comp:340-372)
+ *   SPHERE  in  o(3) d(3) P(3) r              out t (Tracer.comp:314-329)
+ *   PLANE   in  o(3) d(3) N(3) len            out t (Tracer.comp:331-338)
+ *   BRDF    in  N(3) H(3) V(3) L(3) rough cos F0
+ *           out NDF, G, F(3) of fresnel_schlick(cos, F0), schlick(cos, rough) (Tracer.comp:265-312)
+ *   JITTER  in  d(3) phi sina cosa            out vec3 (Tracer.comp:256-263)
+ *   COS     in  x                             out cos(x), sin(x), pow(x, 1/2.2) */
+enum {
+    VKR_DEBUG_RAND = 0, VKR_DEBUG_TRI = 1, VKR_DEBUG_SPHERE = 2, VKR_DEBUG_PLANE = 3,
+    VKR_DEBUG_BRDF = 4, VKR_DEBUG_JITTER = 5, VKR_DEBUG_COS = 6
+};
+vkr_error vkr_debug_eval(vkr_ctx* ctx, uint32_t op, const float* in, float* out, uint32_t n, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Layout checks                                                             */
+/* ------------------------------------------------------------------------ */
+#ifdef __cplusplus
+#define VKR_STATIC_ASSERT(c, m) static_assert(c, m)
+#else
+#define VKR_STATIC_ASSERT(c, m) _Static_assert(c, m)
+#endif
+VKR_STATIC_ASSERT(sizeof(vkr_material) == 48, "std430 Material is 48 bytes");
+VKR_STATIC_ASSERT(offsetof(vkr_material, emissive) == 16, "Material.emissive @16");
+VKR_STATIC_ASSERT(offsetof(vkr_material, roughness) == 28, "Material.roughness @28");
+VKR_STATIC_ASSERT(offsetof(vkr_material, metalness) == 32, "Material.metalness @32");
+VKR_STATIC_ASSERT(offsetof(vkr_material, type) == 36, "Material.type @36");
+VKR_STATIC_ASSERT(sizeof(vkr_triangle) == 48, "std430 Triangle is 48 bytes");
+VKR_STATIC_ASSERT(offsetof(vkr_triangle, v1) == 16, "Triangle.v1 @16");
+VKR_STATIC_ASSERT(offsetof(vkr_triangle, v2) == 32, "Triangle.v2 @32");
+VKR_STATIC_ASSERT(sizeof(vkr_sphere) == 64, "std430 Sphere is 64 bytes");
+VKR_STATIC_ASSERT(offsetof(vkr_sphere, P) == 48, "Sphere.P @48");
+VKR_STATIC_ASSERT(offsetof(vkr_sphere, r) == 60, "Sphere.r @60");
+VKR_STATIC_ASSERT(sizeof(vkr_plane) == 64, "std430 Plane is 64 bytes");
+VKR_STATIC_ASSERT(offsetof(vkr_plane, N) == 48, "Plane.N @48");
+VKR_STATIC_ASSERT(offsetof(vkr_plane, len) == 60, "Plane.len @60");
+VKR_STATIC_ASSERT(sizeof(vkr_camera) == 64, "CameraData is 64 bytes");
+VKR_STATIC_ASSERT(sizeof(vkr_frame_data) == 96, "FrameData is 96 bytes");
+VKR_STATIC_ASSERT(offsetof(vkr_frame_data, seed) == 4, "FrameData.seed @4");
+VKR_STATIC_ASSERT(offsetof(vkr_frame_data, light_pos) == 16, "FrameData.light_pos @16");
+VKR_STATIC_ASSERT(offsetof(vkr_frame_data, camera) == 32, "FrameData.camera @32");
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* VKR_H */

@@ -1,0 +1,50 @@
+"""Helpers shared by the GPU parity tests: run the HIP path through the C ABI
+and compare with the oracle."""
+from __future__ import annotations
+
+import numpy as np
+
+
+def gpu_trace(ctx, sid, wl, sched=0, block=0, chunk=0, flags=0, decisions=None, kernel=0):
+    import torch
+    dev = torch.device("cuda:0")
+    n = wl.width * wl.height
+    out = {
+        "rgba8": torch.zeros((n, 4), dtype=torch.uint8, device=dev),
+        "radiance": torch.zeros((n, 4), dtype=torch.float32, device=dev),
+        "prim_id": torch.zeros(n, dtype=torch.int32, device=dev),
+        "hit_t": torch.zeros(n, dtype=torch.float32, device=dev),
+        "ray_count": torch.zeros(n, dtype=torch.int32, device=dev),
+    }
+    if decisions is not None:
+        out["decisions"] = decisions
+    p = wl.params(sched=sched, block=block, chunk=chunk, flags=flags, kernel=kernel)
+    ctx.trace(sid, wl.frame, p, **out)
+    torch.cuda.synchronize()
+    res = {k: v.cpu().numpy() for k, v in out.items()}
+    res["prim_id"] = res["prim_id"].view(np.uint32)
+    res["ray_count"] = res["ray_count"].view(np.uint32)
+    if "decisions" in res:
+        res["decisions"] = res["decisions"].view(np.uint16)
+    return res
+
+
+def compare(g: dict, r: dict, pixels=None, exclude_x0_width=None) -> dict:
+    """Agreement fractions between GPU outputs g (full frame) and oracle r."""
+    if pixels is not None:
+        g = {k: v[pixels] for k, v in g.items() if k != "decisions"}
+    rad_ok = (np.abs(g["radiance"] - r["radiance"]) <= 1e-4) | (np.isnan(g["radiance"]) & np.isnan(r["radiance"]))
+    t_g, t_r = g["hit_t"], r["hit_t"]
+    t_ok = (t_g.view(np.uint32) == t_r.view(np.uint32)) | (np.abs(t_g - t_r) <= 1e-5 * np.abs(t_r))
+    stats = {
+        "n": int(len(r["prim_id"])),
+        "prim_id": float((g["prim_id"] == r["prim_id"]).mean()),
+        "hit_t_bits": float((t_g.view(np.uint32) == t_r.view(np.uint32)).mean()),
+        "hit_t_1e-5": float(t_ok.mean()),
+        "radiance_1e-4": float(rad_ok.all(axis=1).mean()),
+        "radiance_bits": float((g["radiance"].view(np.uint32) == r["radiance"].view(np.uint32)).all(axis=1).mean()),
+        "rgba8": float((g["rgba8"] == r["rgba8"]).all(axis=1).mean()),
+        "ray_count": float((g["ray_count"] == r["ray_count"]).mean()),
+        "max_rad_err": float(np.nanmax(np.abs(g["radiance"] - r["radiance"]))) if len(r["prim_id"]) else 0.0,
+    }
+    return stats

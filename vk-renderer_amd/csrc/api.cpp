@@ -1,0 +1,539 @@
+// api.cpp - the C ABI of include/vkr.h over the gfx950 kernels.
+//
+// Replaces the compute half of the reference's GraphicsDevice
+// (Source/GraphicsDevice.cpp): device selection, the scene SSBO, the compute
+// pipeline and the per-frame push-constant + dispatch.  No global state: an
+// opaque context owns every device allocation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/vkr.h"
+#include "bvh.hpp"
+#include "kernels.hpp"
+
+using namespace vkr;
+
+namespace {
+
+thread_local std::string g_err = "no error";
+
+vkr_error fail(vkr_error code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+vkr_error fail_hip(const char* what, hipError_t e)
+{
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? VKR_OUT_OF_MEMORY : VKR_HIP_ERROR;
+}
+
+#define VKR_HIP(call)                                    \
+    do {                                                 \
+        const hipError_t e_ = (call);                    \
+        if (e_ != hipSuccess) return fail_hip(#call, e_); \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { reset(); }
+    void reset()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n)
+    {
+        reset();
+        if (n == 0) n = 16;
+        const hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        else p = nullptr;
+        return e;
+    }
+    hipError_t upload(const void* src, size_t n)
+    {
+        const hipError_t e = alloc(n);
+        if (e != hipSuccess) return e;
+        if (n && src) return hipMemcpy(p, src, n, hipMemcpyHostToDevice);
+        return hipSuccess;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+const vkr_material kMirror = {{1.0f, 0.5f, 0.5f}, 0.0f, {0.0f, 0.0f, 0.0f}, 0.0f, 1.0f, VKR_MAT_DIFFUSE, {0.0f, 0.0f}};
+
+struct Scene {
+    DevBuf nodes, tris, sph, sph_key, tris_orig, spheres, planes, tri_mat, materials, emitters;
+    std::vector<vkr_sphere> host_spheres;
+    DevScene dev{};
+    vkr_scene_info info{};
+};
+
+struct Graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    DevBuf jobs, job_start, pool, counters;
+    uint64_t total = 0;
+    ~Graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+}  // namespace
+
+struct vkr_ctx {
+    int device = 0;
+    uint32_t flags = 0;
+    std::vector<std::unique_ptr<Scene>> scenes;
+    std::vector<std::unique_ptr<Graph>> graphs;
+    DevBuf counters, pool, jobs, job_start;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+    DevCounters* last_counters = nullptr;
+};
+
+namespace {
+
+Scene* get_scene(vkr_ctx* ctx, uint32_t id)
+{
+    if (!ctx || id >= ctx->scenes.size()) return nullptr;
+    return ctx->scenes[id].get();
+}
+
+bool nonzero3(const float* v) { return !(v[0] == 0.0f && v[1] == 0.0f && v[2] == 0.0f); }
+
+std::vector<uint32_t> emitter_list(const std::vector<vkr_sphere>& s)
+{
+    // Tracer.comp:458-462: every sphere whose emissive is not vec3(0), in order.
+    std::vector<uint32_t> e;
+    for (uint32_t i = 0; i < s.size(); ++i)
+        if (nonzero3(s[i].mat.emissive)) e.push_back(i);
+    return e;
+}
+
+vkr_error validate_params(const vkr_trace_params* p)
+{
+    if (!p) return fail(VKR_INVALID_ARGUMENT, "params is NULL");
+    if (p->width == 0 || p->height == 0 || p->width > 16384 || p->height > 16384)
+        return fail(VKR_INVALID_ARGUMENT, "image size must be 1..16384 per side");
+    if (p->kernel != VKR_KERNEL_PATH && p->kernel != VKR_KERNEL_WHITTED)
+        return fail(VKR_INVALID_ARGUMENT, "unknown kernel");
+    if (p->kernel == VKR_KERNEL_PATH) {
+        if (p->spp == 0 || p->spp > 4096) return fail(VKR_INVALID_ARGUMENT, "spp must be 1..4096");
+        if (p->max_depth == 0 || p->max_depth > 16) return fail(VKR_INVALID_ARGUMENT, "max_depth must be 1..16");
+        if (!(p->epsilon >= 0.0f) || !std::isfinite(p->epsilon))
+            return fail(VKR_INVALID_ARGUMENT, "epsilon must be finite and >= 0");
+    }
+    if (p->block != 0 && p->block != 64 && p->block != 128 && p->block != 256)
+        return fail(VKR_INVALID_ARGUMENT, "block must be 0, 64, 128 or 256");
+    if (p->chunk > 64) return fail(VKR_INVALID_ARGUMENT, "chunk must be 0..64");
+    if ((p->flags & VKR_TRACE_RECORD_DECISIONS) && (p->flags & VKR_TRACE_REPLAY_DECISIONS))
+        return fail(VKR_INVALID_ARGUMENT, "record and replay decisions are exclusive");
+    return VKR_SUCCESS;
+}
+
+JobDesc make_job(const Scene& sc, const vkr_frame_data& fd, const vkr_trace_params& p, const vkr_outputs* out)
+{
+    JobDesc j{};
+    j.sc = sc.dev;
+    for (int k = 0; k < 3; ++k) {
+        j.cam_pos[k] = fd.camera.pos[k];
+        j.cam_dir[k] = fd.camera.dir[k];
+        j.cam_right[k] = fd.camera.right[k];
+        j.cam_up[k] = fd.camera.up[k];
+        j.light_pos[k] = fd.light_pos[k];
+    }
+    j.aspect = fd.aspect_ratio;
+    j.seed = fd.seed;
+    j.eps = p.epsilon == 0.0f ? 1e-3f : p.epsilon;
+    j.flags = p.flags;
+    j.kernel = p.kernel;
+    j.W = p.width;
+    j.H = p.height;
+    j.spp = p.spp;
+    j.max_depth = p.max_depth;
+    j.tiles_x = (p.width + 7u) / 8u;
+    j.n_tiles = j.tiles_x * ((p.height + 7u) / 8u);
+    if (out) {
+        j.rgba8 = out->rgba8;
+        j.radiance = out->radiance;
+        j.prim_id = out->prim_id;
+        j.hit_t = out->hit_t;
+        j.ray_count = out->ray_count;
+        j.decisions = out->decisions;
+    }
+    return j;
+}
+
+vkr_error ensure_events(vkr_ctx* ctx)
+{
+    if (!ctx->ev0) VKR_HIP(hipEventCreate(&ctx->ev0));
+    if (!ctx->ev1) VKR_HIP(hipEventCreate(&ctx->ev1));
+    return VKR_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* vkr_last_error_string(void) { return g_err.c_str(); }
+
+vkr_error vkr_create(const vkr_create_info* info, vkr_ctx** out_ctx)
+{
+    if (!out_ctx) return fail(VKR_INVALID_ARGUMENT, "out_ctx is NULL");
+    *out_ctx = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return fail(VKR_NO_SUITABLE_GPU, "no HIP device is visible");
+    int dev = 0;
+    if (info && info->device >= 0) dev = info->device;
+    else if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    if (dev >= n) return fail(VKR_NO_SUITABLE_GPU, "device ordinal out of range");
+    hipDeviceProp_t prop;
+    VKR_HIP(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(VKR_NO_SUITABLE_GPU, std::string("kernels are built for gfx950, device is ") + prop.gcnArchName);
+    VKR_HIP(hipSetDevice(dev));
+    auto ctx = std::make_unique<vkr_ctx>();
+    ctx->device = dev;
+    ctx->flags = info ? info->flags : 0u;
+    VKR_HIP(ctx->counters.alloc(sizeof(DevCounters)));
+    VKR_HIP(ctx->pool.alloc(64));
+    VKR_HIP(hipMemset(ctx->counters.p, 0, sizeof(DevCounters)));
+    *out_ctx = ctx.release();
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_destroy(vkr_ctx* ctx)
+{
+    if (!ctx) return VKR_SUCCESS;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    delete ctx;
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_wait_idle(vkr_ctx* ctx)
+{
+    if (!ctx) return fail(VKR_INVALID_ARGUMENT, "ctx is NULL");
+    VKR_HIP(hipSetDevice(ctx->device));
+    VKR_HIP(hipDeviceSynchronize());
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_set_ctx_flags(vkr_ctx* ctx, uint32_t flags)
+{
+    if (!ctx) return fail(VKR_INVALID_ARGUMENT, "ctx is NULL");
+    ctx->flags = flags;
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* out_scene)
+{
+    if (!ctx || !desc || !out_scene) return fail(VKR_INVALID_ARGUMENT, "NULL argument");
+    if ((desc->n_tris && !desc->tris) || (desc->n_spheres && !desc->spheres) || (desc->n_planes && !desc->planes))
+        return fail(VKR_INVALID_ARGUMENT, "primitive array is NULL with a non-zero count");
+    if (desc->n_tris > kRefFirstMask || desc->n_spheres > kRefFirstMask)
+        return fail(VKR_INVALID_ARGUMENT, "too many primitives (limit 2^27 per kind)");
+    if ((desc->flags & VKR_SCENE_TRI_MATERIAL_INDEX) && (!desc->materials || desc->n_materials == 0))
+        return fail(VKR_INVALID_ARGUMENT, "per-triangle materials requested without a material table");
+    if (desc->max_leaf_size > kMaxLeaf) return fail(VKR_INVALID_ARGUMENT, "max_leaf_size must be <= 8");
+    if (desc->max_depth > 64) return fail(VKR_INVALID_ARGUMENT, "max_depth must be <= 64");
+    VKR_HIP(hipSetDevice(ctx->device));
+
+    auto sc = std::make_unique<Scene>();
+    const auto t0 = std::chrono::steady_clock::now();
+    BuildInput bi;
+    bi.tri_v = reinterpret_cast<const float*>(desc->tris);
+    bi.tri_mat = desc->tris ? reinterpret_cast<const uint32_t*>(desc->tris) + 3 : nullptr;
+    bi.n_tris = desc->n_tris;
+    bi.sph = reinterpret_cast<const float*>(desc->spheres);
+    bi.n_spheres = desc->n_spheres;
+    bi.max_leaf = desc->max_leaf_size ? desc->max_leaf_size : 4u;
+    bi.max_depth = desc->max_depth ? desc->max_depth : 32u;
+    BuildOutput bo = build_bvh(bi);
+    const auto t1 = std::chrono::steady_clock::now();
+
+    const uint32_t stack = bo.empty ? 32u : stack_variant(bo.depth);
+    if (stack == 0) return fail(VKR_UNSUPPORTED, "BVH deeper than 64 levels");
+
+    sc->host_spheres.assign(desc->spheres, desc->spheres + desc->n_spheres);
+    const std::vector<uint32_t> em = emitter_list(sc->host_spheres);
+
+    VKR_HIP(sc->nodes.upload(bo.nodes.data(), bo.nodes.size() * sizeof(BvhNode)));
+    VKR_HIP(sc->tris.upload(bo.tris.data(), bo.tris.size() * sizeof(TriRecord)));
+    VKR_HIP(sc->sph.upload(bo.sph.data(), bo.sph.size() * sizeof(float)));
+    VKR_HIP(sc->sph_key.upload(bo.sph_key.data(), bo.sph_key.size() * sizeof(uint32_t)));
+    VKR_HIP(sc->tris_orig.upload(desc->tris, (size_t)desc->n_tris * sizeof(vkr_triangle)));
+    VKR_HIP(sc->spheres.upload(desc->spheres, (size_t)desc->n_spheres * sizeof(vkr_sphere)));
+    VKR_HIP(sc->planes.upload(desc->planes, (size_t)desc->n_planes * sizeof(vkr_plane)));
+    VKR_HIP(sc->tri_mat.upload(desc->tri_material ? desc->tri_material : &kMirror, sizeof(vkr_material)));
+    const uint32_t nm = (desc->flags & VKR_SCENE_TRI_MATERIAL_INDEX) ? desc->n_materials : 0u;
+    VKR_HIP(sc->materials.upload(nm ? desc->materials : nullptr, (size_t)nm * sizeof(vkr_material)));
+    VKR_HIP(sc->emitters.upload(em.data(), em.size() * sizeof(uint32_t)));
+
+    DevScene& d = sc->dev;
+    d.nodes = sc->nodes.as<BvhNode>();
+    d.tris = sc->tris.as<TriRecord>();
+    d.sph = sc->sph.as<float4>();
+    d.sph_key = sc->sph_key.as<uint32_t>();
+    d.tris_orig = sc->tris_orig.as<vkr_triangle>();
+    d.spheres = sc->spheres.as<vkr_sphere>();
+    d.planes = sc->planes.as<vkr_plane>();
+    d.tri_material = sc->tri_mat.as<vkr_material>();
+    d.materials = sc->materials.as<vkr_material>();
+    d.emitters = sc->emitters.as<uint32_t>();
+    d.n_tris = desc->n_tris;
+    d.n_spheres = desc->n_spheres;
+    d.n_planes = desc->n_planes;
+    d.n_emitters = (uint32_t)em.size();
+    d.n_materials = nm;
+    d.flags = desc->flags;
+    d.has_bvh = bo.empty ? 0u : 1u;
+    d.stack_needed = stack;
+
+    vkr_scene_info& in = sc->info;
+    in.n_tris = desc->n_tris;
+    in.n_spheres = desc->n_spheres;
+    in.n_planes = desc->n_planes;
+    in.n_emitters = (uint32_t)em.size();
+    in.n_nodes = (uint32_t)bo.nodes.size();
+    in.n_leaves = bo.leaves;
+    in.depth = bo.depth;
+    in.stack_needed = stack;
+    in.device_bytes = sc->nodes.bytes + sc->tris.bytes + sc->sph.bytes + sc->sph_key.bytes + sc->tris_orig.bytes +
+                      sc->spheres.bytes + sc->planes.bytes + sc->tri_mat.bytes + sc->materials.bytes +
+                      sc->emitters.bytes;
+    in.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+
+    // Reuse a freed slot if any.
+    for (uint32_t i = 0; i < ctx->scenes.size(); ++i) {
+        if (!ctx->scenes[i]) {
+            ctx->scenes[i] = std::move(sc);
+            *out_scene = i;
+            return VKR_SUCCESS;
+        }
+    }
+    ctx->scenes.push_back(std::move(sc));
+    *out_scene = (uint32_t)ctx->scenes.size() - 1u;
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_scene_destroy(vkr_ctx* ctx, uint32_t scene)
+{
+    if (!get_scene(ctx, scene)) return fail(VKR_INVALID_ARGUMENT, "unknown scene");
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    ctx->scenes[scene].reset();
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_scene_get_info(vkr_ctx* ctx, uint32_t scene, vkr_scene_info* out)
+{
+    Scene* s = get_scene(ctx, scene);
+    if (!s || !out) return fail(VKR_INVALID_ARGUMENT, "unknown scene or NULL out");
+    *out = s->info;
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_scene_set_sphere_params(vkr_ctx* ctx, uint32_t scene, const float* params, uint32_t n)
+{
+    Scene* s = get_scene(ctx, scene);
+    if (!s || !params) return fail(VKR_INVALID_ARGUMENT, "unknown scene or NULL params");
+    const uint32_t ns = (uint32_t)s->host_spheres.size();
+    if (n != 7u * ns + 1u) return fail(VKR_INVALID_ARGUMENT, "params must hold 7 * n_spheres + 1 floats");
+    VKR_HIP(hipSetDevice(ctx->device));
+    const float L = params[7u * ns];
+    for (uint32_t i = 0; i < ns; ++i) {
+        vkr_material& m = s->host_spheres[i].mat;
+        const float* q = params + 7u * i;
+        for (int k = 0; k < 3; ++k) {
+            m.albedo[k] = q[k];
+            m.emissive[k] = L * q[3 + k];
+        }
+        m.roughness = q[6];
+    }
+    const std::vector<uint32_t> em = emitter_list(s->host_spheres);
+    VKR_HIP(hipDeviceSynchronize());
+    VKR_HIP(hipMemcpy(s->spheres.p, s->host_spheres.data(), (size_t)ns * sizeof(vkr_sphere), hipMemcpyHostToDevice));
+    if (em.size() * sizeof(uint32_t) > s->emitters.bytes) {
+        VKR_HIP(s->emitters.upload(em.data(), em.size() * sizeof(uint32_t)));
+        s->dev.emitters = s->emitters.as<uint32_t>();
+    } else if (!em.empty()) {
+        VKR_HIP(hipMemcpy(s->emitters.p, em.data(), em.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    s->dev.n_emitters = (uint32_t)em.size();
+    s->info.n_emitters = (uint32_t)em.size();
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_trace_frame(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd, const vkr_trace_params* params,
+                          const vkr_outputs* outputs, void* stream)
+{
+    if (!ctx || !fd || !outputs) return fail(VKR_INVALID_ARGUMENT, "NULL argument");
+    const vkr_error pe = validate_params(params);
+    if (pe != VKR_SUCCESS) return pe;
+    if ((params->flags & (VKR_TRACE_RECORD_DECISIONS | VKR_TRACE_REPLAY_DECISIONS)) && !outputs->decisions)
+        return fail(VKR_INVALID_ARGUMENT, "decision flags need outputs.decisions");
+    if ((params->flags & (VKR_TRACE_RECORD_DECISIONS | VKR_TRACE_REPLAY_DECISIONS)) && params->max_depth > 8)
+        return fail(VKR_INVALID_ARGUMENT, "decision record/replay supports max_depth <= 8");
+    Scene* s = nullptr;
+    if (params->kernel == VKR_KERNEL_PATH) {
+        s = get_scene(ctx, scene);
+        if (!s) return fail(VKR_INVALID_ARGUMENT, "unknown scene");
+    }
+    VKR_HIP(hipSetDevice(ctx->device));
+    const vkr_error ee = ensure_events(ctx);
+    if (ee != VKR_SUCCESS) return ee;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    static const Scene kNoScene{};
+    JobDesc j = make_job(s ? *s : kNoScene, *fd, *params, outputs);
+    DevCounters* cnt = ctx->counters.as<DevCounters>();
+    uint32_t* pool = ctx->pool.as<uint32_t>();
+    VKR_HIP(launch_zero_counters(cnt, pool, st));
+    VKR_HIP(hipEventRecord(ctx->ev0, st));
+    if (params->kernel == VKR_KERNEL_WHITTED) {
+        VKR_HIP(launch_whitted(j, cnt, st));
+    } else {
+        PathLaunch a{};
+        a.job = j;
+        a.total = j.n_tiles * 64u;
+        a.pool = pool;
+        a.counters = cnt;
+        const uint32_t block = params->block ? params->block : 128u;
+        const bool persistent = params->sched != VKR_SCHED_PLAIN;
+        a.persistent = persistent ? 1u : 0u;
+        a.chunk = params->chunk ? params->chunk : 32u;
+        const bool count = (ctx->flags & VKR_CTX_COUNT_WORK) != 0;
+        const uint32_t stack = s->dev.stack_needed;
+        const uint32_t grid = persistent ? std::min(persistent_grid(block, stack, count, false, ctx->device),
+                                                    (a.total + block - 1u) / block)
+                                         : (a.total + block - 1u) / block;
+        VKR_HIP(launch_path(a, block, stack, count, false, grid, st));
+    }
+    VKR_HIP(hipEventRecord(ctx->ev1, st));
+    ctx->timed = true;
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_last_counters(vkr_ctx* ctx, vkr_counters* out)
+{
+    if (!ctx || !out) return fail(VKR_INVALID_ARGUMENT, "NULL argument");
+    VKR_HIP(hipSetDevice(ctx->device));
+    if (ctx->timed) VKR_HIP(hipEventSynchronize(ctx->ev1));
+    DevCounters c{};
+    const void* src = ctx->last_counters ? (const void*)ctx->last_counters : ctx->counters.p;
+    VKR_HIP(hipMemcpy(&c, src, sizeof(c), hipMemcpyDeviceToHost));
+    out->rays = c.rays;
+    out->shadow_rays = c.shadow;
+    out->nodes_visited = c.nodes;
+    out->tris_tested = c.tris;
+    out->spheres_tested = c.spheres;
+    out->planes_tested = c.planes;
+    out->window_overflow = c.overflow;
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_last_kernel_ms(vkr_ctx* ctx, float* out_ms)
+{
+    if (!ctx || !out_ms) return fail(VKR_INVALID_ARGUMENT, "NULL argument");
+    if (!ctx->timed) return fail(VKR_INVALID_ARGUMENT, "nothing launched yet");
+    VKR_HIP(hipSetDevice(ctx->device));
+    VKR_HIP(hipEventSynchronize(ctx->ev1));
+    VKR_HIP(hipEventElapsedTime(out_ms, ctx->ev0, ctx->ev1));
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_filter_frame(vkr_ctx* ctx, const uint8_t* current, const uint8_t* previous, uint32_t src_w,
+                           uint32_t src_h, uint8_t* out, uint32_t dst_w, uint32_t dst_h, void* stream)
+{
+    if (!ctx || !current || !previous || !out) return fail(VKR_INVALID_ARGUMENT, "NULL argument");
+    if (!src_w || !src_h || !dst_w || !dst_h || src_w > 16384 || src_h > 16384 || dst_w > 16384 || dst_h > 16384)
+        return fail(VKR_INVALID_ARGUMENT, "image sizes must be 1..16384");
+    VKR_HIP(hipSetDevice(ctx->device));
+    VKR_HIP(launch_filter(current, previous, src_w, src_h, out, dst_w, dst_h, static_cast<hipStream_t>(stream)));
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_debug_eval(vkr_ctx* ctx, uint32_t op, const float* in, float* out, uint32_t n, void* stream)
+{
+    if (!ctx || !in || !out) return fail(VKR_INVALID_ARGUMENT, "NULL argument");
+    if (op > VKR_DEBUG_COS) return fail(VKR_INVALID_ARGUMENT, "unknown debug op");
+    if (n == 0) return VKR_SUCCESS;
+    VKR_HIP(hipSetDevice(ctx->device));
+    VKR_HIP(launch_debug(op, in, out, n, static_cast<hipStream_t>(stream)));
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_camera_from_angles(const float pos[3], float pitch, float yaw, vkr_camera* out)
+{
+    // Camera::update (Source/Camera.cpp:91-107) with GLM 0.9.9 float
+    // semantics: radians(x) = x * float(pi / 180), normalize(v) =
+    // v * (1 / sqrt(dot(v, v))), dot = (x*x' + y*y') + z*z'.
+    if (!pos || !out) return fail(VKR_INVALID_ARGUMENT, "NULL argument");
+    struct V3 { float x, y, z; };
+    auto dot3 = [](V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; };
+    auto nrm = [&](V3 a) {
+        const float inv = 1.0f / std::sqrt(dot3(a, a));
+        return V3{a.x * inv, a.y * inv, a.z * inv};
+    };
+    auto crs = [](V3 a, V3 b) {
+        return V3{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+    };
+    const float p = std::min(std::max(pitch, -89.0f), 89.0f);
+    const float deg = 0.01745329251994329576923690768489f;
+    const float pr = p * deg, yr = yaw * deg;
+    V3 front{std::cos(pr) * std::cos(yr), std::sin(pr), std::cos(pr) * std::sin(yr)};
+    front = nrm(front);
+    const V3 P{pos[0], pos[1], pos[2]};
+    const V3 moved{P.x + front.x, P.y + front.y, P.z + front.z};
+    const V3 dir = nrm(V3{moved.x - P.x, moved.y - P.y, moved.z - P.z});
+    const V3 right = nrm(crs(V3{0.0f, 1.0f, 0.0f}, dir));
+    const V3 up = crs(dir, right);
+    std::memset(out, 0, sizeof(*out));
+    out->pos[0] = P.x; out->pos[1] = P.y; out->pos[2] = P.z;
+    out->dir[0] = dir.x; out->dir[1] = dir.y; out->dir[2] = dir.z;
+    out->right[0] = right.x; out->right[1] = right.y; out->right[2] = right.z;
+    out->up[0] = up.x; out->up[1] = up.y; out->up[2] = up.z;
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_trace_batch(vkr_ctx*, const vkr_scene_job*, uint32_t, uint32_t, uint32_t, void*)
+{
+    return fail(VKR_UNSUPPORTED, "batched tracing is not built yet");
+}
+
+vkr_error vkr_batch_graph_create(vkr_ctx*, const vkr_scene_job*, uint32_t, uint32_t, uint32_t, void*, uint32_t*)
+{
+    return fail(VKR_UNSUPPORTED, "batched tracing is not built yet");
+}
+
+vkr_error vkr_graph_launch(vkr_ctx*, uint32_t, void*) { return fail(VKR_UNSUPPORTED, "no graphs"); }
+vkr_error vkr_graph_destroy(vkr_ctx*, uint32_t) { return fail(VKR_UNSUPPORTED, "no graphs"); }
+
+vkr_error vkr_adjoint(vkr_ctx*, uint32_t, const vkr_frame_data*, const vkr_trace_params*, const float*, float*,
+                      void*)
+{
+    return fail(VKR_UNSUPPORTED, "adjoint is not built yet");
+}
+
+}  // extern "C"

@@ -1,0 +1,73 @@
+// bvh.hpp - host-side BVH builder for the gfx950 traversal kernels.
+//
+// The reference has no acceleration structure: trace_ray scans every
+// primitive linearly (Assets/Tracer.comp:374-431).  This BVH replaces that
+// scan; the kernels recover the scan's exact order-dependent acceptance rule
+// from the hits it reports (see trace_kernels.hip).
+//
+// Layout (all records 16-byte aligned for 128-bit loads):
+//   Node (64 B): the bounds of BOTH children plus their refs, so one node
+//   fetch tests two boxes:
+//     a = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
+//     b = (c0.lo.z, c0.hi.z, c1.lo.x, c1.hi.x)
+//     c = (c1.lo.y, c1.hi.y, c1.lo.z, c1.hi.z)
+//     d = (ref0, ref1, 0, 0)
+//   ref: bit31 = leaf; for a leaf bit30 = sphere leaf, bits 27..29 = count-1,
+//   bits 0..26 = first record; for an inner node the node index.
+//   Triangle record (48 B): (v0, key) (e1, mat_index) (e2, 0) where
+//   e1 = fl(v1 - v0), e2 = fl(v2 - v0) are exactly the edges the reference
+//   computes (Tracer.comp:342-343) and key is the triangle's scan index.
+//   Sphere record (16 B + 4 B key): (centre, r) and the sphere's scan index.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace vkr {
+
+constexpr uint32_t kRefLeaf = 0x80000000u;
+constexpr uint32_t kRefSphere = 0x40000000u;
+constexpr uint32_t kRefCountShift = 27;
+constexpr uint32_t kRefFirstMask = (1u << 27) - 1u;
+constexpr uint32_t kMaxLeaf = 8;
+
+struct alignas(16) BvhNode {
+    float a[4], b[4], c[4];
+    uint32_t d[4];
+};
+static_assert(sizeof(BvhNode) == 64, "node is 64 bytes");
+
+struct alignas(16) TriRecord {
+    float v0[3]; uint32_t key;
+    float e1[3]; uint32_t mat_index;
+    float e2[3]; uint32_t pad;
+};
+static_assert(sizeof(TriRecord) == 48, "triangle record is 48 bytes");
+
+struct BuildInput {
+    const float* tri_v;       // n_tris * 12 floats: v0(3) pad v1(3) pad v2(3) pad (std430)
+    const uint32_t* tri_mat;  // optional, stride 12 u32 (the v0 padding word)
+    uint32_t n_tris = 0;
+    const float* sph;         // n_spheres * 16 floats (std430 Sphere: P at 12, r at 15)
+    uint32_t n_spheres = 0;
+    uint32_t max_leaf = 4;
+    uint32_t max_depth = 32;
+};
+
+struct BuildOutput {
+    std::vector<BvhNode> nodes;
+    std::vector<TriRecord> tris;
+    std::vector<float> sph;        // 4 floats per sphere record (centre, r)
+    std::vector<uint32_t> sph_key; // scan index per sphere record
+    uint32_t leaves = 0;
+    uint32_t depth = 0;            // deepest leaf (root children are depth 1)
+    bool empty = true;
+};
+
+// Builds a binned-SAH BVH.  Leaves hold one primitive kind only and at most
+// max_leaf primitives; no leaf is deeper than max_depth (splits fall back to
+// balanced median splits near the cap), so a traversal stack of max_depth
+// entries never overflows.
+BuildOutput build_bvh(const BuildInput& in);
+
+}  // namespace vkr

@@ -1,0 +1,91 @@
+// kernels.hpp - device data layout shared by the kernels and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/vkr.h"
+#include "bvh.hpp"
+
+namespace vkr {
+
+constexpr uint32_t kMiss = 0xFFFFFFFFu;
+
+// Everything a kernel needs of one uploaded scene (device pointers).
+struct DevScene {
+    const BvhNode* nodes;
+    const TriRecord* tris;          // leaf order
+    const float4* sph;              // leaf order: centre, r
+    const uint32_t* sph_key;        // leaf order: sphere scan index
+    const vkr_triangle* tris_orig;  // scan order (std430, as uploaded)
+    const vkr_sphere* spheres;      // scan order
+    const vkr_plane* planes;        // scan order
+    const vkr_material* tri_material;
+    const vkr_material* materials;
+    const uint32_t* emitters;       // scan indices of spheres with emissive != 0
+    uint32_t n_tris, n_spheres, n_planes, n_emitters;
+    uint32_t n_materials, flags, has_bvh, stack_needed;
+};
+
+// One frame of work (a single trace call, or one job of a batch).
+struct JobDesc {
+    DevScene sc;
+    float cam_pos[3];
+    float aspect;
+    float cam_dir[3];
+    float seed;
+    float cam_right[3];
+    float eps;
+    float cam_up[3];
+    uint32_t flags;               // VKR_TRACE_*
+    float light_pos[3];           // FrameData.light_pos (read by the Whitted kernel only)
+    uint32_t kernel;              // VKR_KERNEL_*
+    uint32_t W, H, spp, max_depth;
+    uint32_t tiles_x, n_tiles, pool_base, pad;
+    uint8_t* rgba8;
+    float* radiance;
+    uint32_t* prim_id;
+    float* hit_t;
+    uint32_t* ray_count;
+    uint16_t* decisions;
+};
+
+struct DevCounters {
+    unsigned long long rays, shadow, nodes, tris, spheres, planes, overflow;
+};
+
+struct PathLaunch {
+    JobDesc job;                  // single-frame launches
+    const JobDesc* jobs;          // batched launches
+    const uint32_t* job_start;    // n_jobs + 1 pool offsets (batched)
+    uint32_t n_jobs;
+    uint32_t total;               // pool indices (64 per 8x8 tile)
+    uint32_t persistent;          // 0: one pixel per lane; 1: refill from pool
+    uint32_t chunk;               // refill when at least this many lanes idle
+    uint32_t* pool;               // persistent pool cursor (device)
+    DevCounters* counters;
+};
+
+// Adjoint launch (path replay), see adjoint.hip.
+struct AdjointLaunch {
+    JobDesc job;
+    const float* d_radiance;      // W*H*4
+    float* d_params;              // 7 * n_spheres + 1
+    const float* params;          // 7 * n_spheres + 1 (current values)
+    DevCounters* counters;
+};
+
+hipError_t launch_path(const PathLaunch& a, uint32_t block, uint32_t stack, bool count, bool batch,
+                       uint32_t grid, hipStream_t s);
+hipError_t launch_whitted(const JobDesc& j, DevCounters* c, hipStream_t s);
+hipError_t launch_filter(const uint8_t* cur, const uint8_t* prev, uint32_t sw, uint32_t sh, uint8_t* out,
+                         uint32_t dw, uint32_t dh, hipStream_t s);
+hipError_t launch_adjoint(const AdjointLaunch& a, uint32_t stack, hipStream_t s);
+hipError_t launch_debug(uint32_t op, const float* in, float* out, uint32_t n, hipStream_t s);
+hipError_t launch_zero_counters(DevCounters* c, uint32_t* pool, hipStream_t s);
+// Occupancy-derived persistent grid size for (block, stack).
+uint32_t persistent_grid(uint32_t block, uint32_t stack, bool count, bool batch, int device);
+// Largest stack depth with a compiled kernel variant (0 when none fits).
+uint32_t stack_variant(uint32_t needed);
+
+}  // namespace vkr

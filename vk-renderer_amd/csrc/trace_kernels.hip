@@ -1,0 +1,890 @@
+// trace_kernels.hip - gfx950 kernels for the vk-renderer compute path tracer.
+//
+// The reference (Assets/Tracer.comp) runs one invocation per pixel that
+// loops SAMPLES paths of up to DEPTH bounces and scans every primitive
+// linearly per ray.  Here:
+//   * each lane runs one pixel as a small state machine; every loop
+//     iteration traces exactly one ray per lane (a path segment or one
+//     next-event shadow ray), so all lanes of a wave traverse together;
+//   * persistent waves refill idle lanes from a global pixel pool with one
+//     wave-wide ballot + one atomic per refill (8x8 tiles in pool order);
+//   * the linear scan is replaced by a BVH (two child boxes per 64-byte node,
+//     128-bit loads) with a short per-lane stack in LDS;
+//   * the reference's order-dependent acceptance rule
+//     (`t > EPSILON && t < best + EPSILON` for triangles then spheres,
+//     `t < best - EPSILON` for planes, Tracer.comp:378-428) is reproduced
+//     exactly: the smallest hit t* is always accepted by the scan, so only
+//     hits within a window above t* can still change the winner; they are
+//     kept in a 4-entry list and replayed in scan order after traversal.
+//     A window or list overflow is counted (documented near-tie).
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "vkr_device.hpp"
+
+namespace vkr {
+
+constexpr uint32_t kDone = 0xFFFFFFFFu;
+constexpr uint32_t kPhTrace = 0;
+constexpr uint32_t kPhNee = 1;
+constexpr int kCand = 4;
+
+struct WorkCnt {
+    uint32_t nodes, tris, spheres, planes, overflow;
+};
+
+struct TravOut {
+    uint32_t prim;  // (kind << 30) | index, kMiss when nothing was accepted
+    float t;        // accepted t (t_max when missed)
+    bool hit;       // closest: something accepted; any: occluded
+};
+
+__device__ __forceinline__ void cswap(float& ta, uint32_t& ka, float& tb, uint32_t& kb)
+{
+    const bool s = kb < ka;
+    const float t0 = s ? tb : ta, t1 = s ? ta : tb;
+    const uint32_t k0 = s ? kb : ka, k1 = s ? ka : kb;
+    ta = t0; tb = t1; ka = k0; kb = k1;
+}
+
+// trace_ray of Assets/Tracer.comp:374-431 over the BVH.
+//   any == false: closest hit with the reference's acceptance rule.
+//   any == true : only "was anything accepted" (shadow rays, :473).
+template <int BLOCK, bool COUNT>
+__device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, float tmax, float eps, bool any,
+                                            uint32_t* stk, WorkCnt& wc)
+{
+    const float tmax_eps = tmax + eps;
+    const float win = 4.0f * eps;
+    const float inf = __builtin_huge_valf();
+    float ct0 = inf, ct1 = inf, ct2 = inf, ct3 = inf;
+    uint32_t ck0 = 0, ck1 = 0, ck2 = 0, ck3 = 0;
+    float tmin = inf;
+    bool ovf = false;
+    bool occluded = false;
+    float cull = any ? tmax_eps : tmax_eps + win;
+
+    const bool nan_ray = (d.x != d.x) || (d.y != d.y) || (d.z != d.z) || (o.x != o.x) || (o.y != o.y) || (o.z != o.z);
+    const BvhNode* __restrict__ nodes = sc.nodes;
+    const TriRecord* __restrict__ tris = sc.tris;
+    const float4* __restrict__ sph = sc.sph;
+    const uint32_t* __restrict__ sph_key = sc.sph_key;
+
+    if (sc.has_bvh && !nan_ray) {
+        const float ix = 1.0f / (fabsf(d.x) > 1e-20f ? d.x : copysignf(1e-20f, d.x));
+        const float iy = 1.0f / (fabsf(d.y) > 1e-20f ? d.y : copysignf(1e-20f, d.y));
+        const float iz = 1.0f / (fabsf(d.z) > 1e-20f ? d.z : copysignf(1e-20f, d.z));
+        const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
+        uint32_t ref = 0;
+        int sp = 0;
+        while (ref != kDone) {
+            while (!(ref & kRefLeaf)) {
+                const float4* np = reinterpret_cast<const float4*>(nodes + ref);
+                const float4 A = np[0], B = np[1], C = np[2];
+                const uint4 D = reinterpret_cast<const uint4*>(np)[3];
+                if (COUNT) wc.nodes++;
+                const float cb = cull + fabsf(cull) * 1e-5f + 1e-5f;
+                const float a0 = __builtin_fmaf(A.x, ix, -ox), a1 = __builtin_fmaf(A.y, ix, -ox);
+                const float a2 = __builtin_fmaf(A.z, iy, -oy), a3 = __builtin_fmaf(A.w, iy, -oy);
+                const float a4 = __builtin_fmaf(B.x, iz, -oz), a5 = __builtin_fmaf(B.y, iz, -oz);
+                const float b0 = __builtin_fmaf(B.z, ix, -ox), b1 = __builtin_fmaf(B.w, ix, -ox);
+                const float b2 = __builtin_fmaf(C.x, iy, -oy), b3 = __builtin_fmaf(C.y, iy, -oy);
+                const float b4 = __builtin_fmaf(C.z, iz, -oz), b5 = __builtin_fmaf(C.w, iz, -oz);
+                const float n0 = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), 0.0f));
+                const float f0 = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), cb));
+                const float n1 = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), 0.0f));
+                const float f1 = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), cb));
+                const bool h0 = n0 <= f0, h1 = n1 <= f1;
+                if (h0 && h1) {
+                    const bool first0 = n0 <= n1;
+                    const uint32_t nearr = first0 ? D.x : D.y;
+                    const uint32_t farr = first0 ? D.y : D.x;
+                    stk[sp * BLOCK] = farr;
+                    ++sp;
+                    ref = nearr;
+                } else if (h0) {
+                    ref = D.x;
+                } else if (h1) {
+                    ref = D.y;
+                } else {
+                    ref = sp > 0 ? stk[(sp - 1) * BLOCK] : kDone;
+                    sp = sp > 0 ? sp - 1 : 0;
+                }
+            }
+            if (ref == kDone) break;
+
+            // Leaf: test its primitives.
+            const uint32_t count = ((ref >> kRefCountShift) & 7u) + 1u;
+            const uint32_t first = ref & kRefFirstMask;
+            for (uint32_t i = 0; i < count; ++i) {
+                float t;
+                uint32_t key;
+                if (ref & kRefSphere) {
+                    const float4 s4 = sph[first + i];
+                    if (COUNT) wc.spheres++;
+                    t = sphere_t(o, d, f3{s4.x, s4.y, s4.z}, s4.w);
+                    key = 0x80000000u | sph_key[first + i];
+                } else {
+                    const float4* tp = reinterpret_cast<const float4*>(tris + first + i);
+                    const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+                    if (COUNT) wc.tris++;
+                    t = tri_t(o, d, f3{p0.x, p0.y, p0.z}, f3{p1.x, p1.y, p1.z}, f3{p2.x, p2.y, p2.z}, eps);
+                    key = __float_as_uint(p0.w);
+                }
+                if (!(t > eps)) continue;
+                if (any) {
+                    if (t < tmax_eps) occluded = true;
+                } else if (t < fminf(tmin, tmax_eps) + win) {
+                    if (t < tmin) {
+                        tmin = t;
+                        const float lim = tmin + win;
+                        ct0 = ct0 < lim ? ct0 : inf;
+                        ct1 = ct1 < lim ? ct1 : inf;
+                        ct2 = ct2 < lim ? ct2 : inf;
+                        ct3 = ct3 < lim ? ct3 : inf;
+                    }
+                    if (ct0 == inf) { ct0 = t; ck0 = key; }
+                    else if (ct1 == inf) { ct1 = t; ck1 = key; }
+                    else if (ct2 == inf) { ct2 = t; ck2 = key; }
+                    else if (ct3 == inf) { ct3 = t; ck3 = key; }
+                    else {
+                        // List full: keep the four smallest t (documented near-tie).
+                        ovf = true;
+                        float mx = ct0; int mi = 0;
+                        if (ct1 > mx) { mx = ct1; mi = 1; }
+                        if (ct2 > mx) { mx = ct2; mi = 2; }
+                        if (ct3 > mx) { mx = ct3; mi = 3; }
+                        if (t < mx) {
+                            if (mi == 0) { ct0 = t; ck0 = key; }
+                            else if (mi == 1) { ct1 = t; ck1 = key; }
+                            else if (mi == 2) { ct2 = t; ck2 = key; }
+                            else { ct3 = t; ck3 = key; }
+                        }
+                    }
+                    cull = fminf(tmin, tmax_eps) + win;
+                }
+            }
+            if (occluded) break;
+            ref = sp > 0 ? stk[(sp - 1) * BLOCK] : kDone;
+            sp = sp > 0 ? sp - 1 : 0;
+        }
+    }
+
+    TravOut r;
+    const uint32_t np = sc.n_planes;
+    const vkr_plane* __restrict__ planes = sc.planes;
+    if (any) {
+        if (!occluded && !nan_ray) {
+            const float lim = tmax - eps;
+            for (uint32_t i = 0; i < np; ++i) {
+                if (COUNT) wc.planes++;
+                const float t = plane_t(o, d, ld3(planes[i].N), planes[i].len);
+                if ((t > eps) && (t < lim)) { occluded = true; break; }
+            }
+        }
+        r.hit = occluded;
+        r.prim = kMiss;
+        r.t = tmax;
+        return r;
+    }
+    // Replay the scan's sequential rule over the candidates in scan order.
+    cswap(ct0, ck0, ct1, ck1);
+    cswap(ct2, ck2, ct3, ck3);
+    cswap(ct0, ck0, ct2, ck2);
+    cswap(ct1, ck1, ct3, ck3);
+    cswap(ct1, ck1, ct2, ck2);
+    float best = tmax;
+    uint32_t win_key = kMiss;
+    const float top = tmin + win - eps;
+#define VKR_REPLAY(T, K)                          \
+    if ((T) < best + eps) {                       \
+        best = (T);                               \
+        win_key = (K);                            \
+        if (best >= top) ovf = true;              \
+    }
+    VKR_REPLAY(ct0, ck0)
+    VKR_REPLAY(ct1, ck1)
+    VKR_REPLAY(ct2, ck2)
+    VKR_REPLAY(ct3, ck3)
+#undef VKR_REPLAY
+    uint32_t prim = kMiss;
+    if (win_key != kMiss) prim = (win_key & 0x80000000u) ? ((1u << 30) | (win_key & 0x7FFFFFFFu)) : win_key;
+    if (!nan_ray) {
+        for (uint32_t i = 0; i < np; ++i) {
+            if (COUNT) wc.planes++;
+            const float t = plane_t(o, d, ld3(planes[i].N), planes[i].len);
+            if ((t > eps) && (t < best - eps)) {
+                best = t;
+                prim = (2u << 30) | i;
+            }
+        }
+    }
+    if (ovf) wc.overflow++;
+    r.hit = prim != kMiss;
+    r.prim = prim;
+    r.t = best;
+    return r;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void flush_counter(unsigned long long* dst, uint32_t v)
+{
+    const uint32_t s = wave_sum(v);
+    if (__lane_id() == 0 && s) atomicAdd(dst, (unsigned long long)s);
+}
+
+// XCD-aware remap: consecutive workgroup ids land on different XCDs, so give
+// each XCD a contiguous range of tiles (its L2 then sees neighbouring rays).
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t g)
+{
+    const uint32_t q = g / 8u, r = g % 8u;
+    const uint32_t x = b % 8u, i = b / 8u;
+    return (x < r) ? x * (q + 1u) + i : r * (q + 1u) + (x - r) * q + i;
+}
+
+// The path program of Assets/Tracer.comp:433-593 as a per-lane state machine.
+// Each loop iteration: (1) refill idle lanes, (2) set up this lane's ray
+// (a path segment, or the shadow ray of the next emitter), (3) traverse,
+// (4) shade, then run the shared tail blocks (finish diffuse -> Russian
+// roulette -> end of path -> end of pixel) at most once each.
+template <int BLOCK, int STACK, bool COUNT, bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
+{
+    __shared__ uint32_t s_stack[STACK * BLOCK];
+    uint32_t* stk = s_stack + threadIdx.x;
+
+    // Lane state.
+    bool has = false;
+    bool exhausted = !A.persistent;
+    uint32_t job = 0, px = 0, py = 0;
+    float uvx = 0.0f, uvy = 0.0f, salt = 0.0f;
+    uint32_t smp = 0, depth = 0, phase = kPhTrace;
+    f3 ro{}, rd{}, acc{}, mask{}, accum{};
+    f3 P{}, N{}, dj{}, e{}, em{}, alb{}, Lsh{};
+    float rough = 0.0f, metal = 0.0f, tsh = 0.0f;
+    uint32_t ei = 0, sidx = 0;
+    uint32_t rays_px = 0, prim0 = kMiss, drec = 0, drep = 0;
+    float t0 = 0.0f;
+    uint32_t c_rays = 0, c_shadow = 0;
+    bool new_path = false;
+    WorkCnt wc{0, 0, 0, 0, 0};
+
+    // Claims pool index g for this lane (no-op for indices outside the image).
+    auto claim = [&](uint32_t g) __attribute__((always_inline)) {
+        uint32_t local = g;
+        if constexpr (BATCH) {
+            uint32_t lo = 0, hi = A.n_jobs;
+            while (hi - lo > 1u) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (A.job_start[mid] <= g) lo = mid; else hi = mid;
+            }
+            job = lo;
+            local = g - A.job_start[lo];
+        }
+        const JobDesc& j = BATCH ? A.jobs[job] : A.job;
+        const uint32_t tile = local >> 6, within = local & 63u;
+        if (tile >= j.n_tiles) return;
+        const uint32_t x = (tile % j.tiles_x) * 8u + (within & 7u);
+        const uint32_t y = (tile / j.tiles_x) * 8u + (within >> 3);
+        if (x >= j.W || y >= j.H) return;
+        px = x;
+        py = y;
+        has = true;
+        uvx = (float)x / (float)j.W;                 // :563
+        uvy = (float)y / (float)j.H;
+        salt = uvy / uvx + j.seed;                   // :565-566
+        smp = 0;
+        accum = f3{0.0f, 0.0f, 0.0f};
+        rays_px = 0;
+        prim0 = kMiss;
+        t0 = 0.0f;
+        new_path = true;
+    };
+
+    if (!A.persistent) {
+        const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+        const uint32_t g = b * BLOCK + threadIdx.x;
+        if (g < A.total) claim(g);
+    }
+
+    while (true) {
+        if (A.persistent && !exhausted) {
+            const unsigned long long need = __ballot(!has);
+            const unsigned long long busy = __ballot(has);
+            const uint32_t nneed = (uint32_t)__popcll(need);
+            if (nneed > 0 && (nneed >= A.chunk || busy == 0)) {
+                const int leader = __ffsll((long long)need) - 1;
+                uint32_t base = 0;
+                if (__lane_id() == leader) base = atomicAdd(A.pool, nneed);
+                base = __shfl(base, leader, 64);
+                if (base + nneed >= A.total) exhausted = true;
+                if (!has) {
+                    const uint32_t rank = (uint32_t)__popcll(need & ((1ull << __lane_id()) - 1ull));
+                    const uint32_t g = base + rank;
+                    if (g < A.total) claim(g);
+                }
+            }
+        }
+        const unsigned long long live = __ballot(has);
+        if (live == 0) {
+            if (exhausted) break;
+            continue;
+        }
+
+        const JobDesc& j = BATCH ? A.jobs[job] : A.job;
+        if (has && new_path) {
+            // :561-574 primary ray; every sample restarts from it (:578-581).
+            new_path = false;
+            const float tx = 2.0f * uvx - 1.0f;
+            const float ty = 2.0f * uvy - 1.0f;
+            f3 dir = add(add(ld3(j.cam_dir), muls(ld3(j.cam_right), tx)), muls(ld3(j.cam_up), ty));
+            dir = mul(dir, f3{j.aspect, 1.0f, j.aspect});
+            ro = ld3(j.cam_pos);
+            rd = normalize(dir);
+            acc = f3{0.0f, 0.0f, 0.0f};
+            mask = f3{1.0f, 1.0f, 1.0f};
+            depth = 0;
+            phase = kPhTrace;
+            drec = 0;
+            drep = 0;
+            if ((j.flags & VKR_TRACE_REPLAY_DECISIONS) && j.decisions)
+                drep = j.decisions[((size_t)py * j.W + px) * j.spp + smp];
+        }
+
+        f3 o{}, d{};
+        float tmax = 0.0f;
+        bool any = false;
+        if (has) {
+            if (phase == kPhTrace) {
+                acc = clamps(acc, 0.0f, 1.0f);                      // :441
+                const float dp1 = (float)(depth + 1u);
+                tmax = 3000.0f / (dp1 * dp1);                        // :444
+                o = ro;
+                d = rd;
+            } else {
+                // Next-event sample of emitter ei, :464-469.
+                sidx = j.sc.emitters[ei];
+                const vkr_sphere* S = j.sc.spheres + sidx;
+                const f3 sP = ld3(S->P);
+                const float sr = S->r;
+                tsh = length(sub(sP, P)) - sr;
+                const f3 l0 = sub(sP, P);
+                const float cos_a_max = sqrtf(1.0f - gclamp(sr * sr / dot(l0, l0), 0.0f, 1.0f));
+                const float cosa = gmix(cos_a_max, 1.0f, rng_next(salt, uvx, uvy));
+                const float phi = kTwoPi * rng_next(salt, uvx, uvy);
+                Lsh = jitter(l0, phi, sqrtf(1.0f - cosa * cosa), cosa);
+                o = P;
+                d = Lsh;
+                tmax = tsh;
+                any = true;
+                ++c_shadow;
+            }
+            ++rays_px;
+            ++c_rays;
+        }
+
+        TravOut tr{kMiss, 0.0f, false};
+        if (has) tr = traverse<BLOCK, COUNT>(j.sc, o, d, tmax, j.eps, any, stk, wc);
+        if (!has) continue;
+
+        bool fin_diffuse = false, do_rr = false, path_end = false;
+        if (phase == kPhTrace) {
+            if (smp == 0 && depth == 0) { prim0 = tr.prim; t0 = tr.t; }
+            if (!tr.hit) {
+                path_end = true;                                     // :445
+            } else {
+                P = add(ro, muls(rd, tr.t));
+                const uint32_t kind = tr.prim >> 30, idx = tr.prim & 0x3FFFFFFFu;
+                const vkr_material* m;
+                if (kind == 0) {
+                    const vkr_triangle* T = j.sc.tris_orig + idx;
+                    const f3 v0 = ld3(T->v0), v1 = ld3(T->v1), v2 = ld3(T->v2);
+                    const f3 u = sub(v1, v0), v = sub(v2, v0);
+                    N = f3{(u.y * v.z) - (u.z * v.y), (u.z * v.x) - (u.x * v.z), (u.x * v.y) - (u.y * v.x)};  // :392
+                    const uint32_t mi = T->mat_index;
+                    m = ((j.sc.flags & VKR_SCENE_TRI_MATERIAL_INDEX) && mi < j.sc.n_materials) ? j.sc.materials + mi
+                                                                                               : j.sc.tri_material;
+                } else if (kind == 1) {
+                    const vkr_sphere* S = j.sc.spheres + idx;
+                    N = divs(sub(P, ld3(S->P)), S->r);               // :408
+                    m = &S->mat;
+                } else {
+                    const vkr_plane* Q = j.sc.planes + idx;
+                    N = ld3(Q->N);                                   // :424
+                    m = &Q->mat;
+                }
+                alb = ld3(m->albedo);
+                const f3 emis = ld3(m->emissive);
+                rough = m->roughness;
+                metal = m->metalness;
+                const uint32_t type = m->type;
+                if (type == VKR_MAT_DIFFUSE) {                       // :451-513
+                    const float r2 = rng_next(salt, uvx, uvy);
+                    const float phi = kTwoPi * rng_next(salt, uvx, uvy);
+                    dj = muls(jitter(N, phi, sqrtf(r2), sqrtf(1.0f - r2)), 1.0f - metal);
+                    e = f3{0.0f, 0.0f, 0.0f};
+                    const bool all_pos = gmax(gsign(emis.x), 0.0f) == 1.0f && gmax(gsign(emis.y), 0.0f) == 1.0f &&
+                                         gmax(gsign(emis.z), 0.0f) == 1.0f;
+                    em = all_pos ? normalize(emis) : f3{0.0f, 0.0f, 0.0f};  // :507
+                    ei = 0;
+                    if (j.sc.n_emitters > 0) phase = kPhNee;
+                    else fin_diffuse = true;
+                } else if (type == VKR_MAT_DIELECTRIC) {             // :514-542
+                    acc = add(acc, mul(emis, mask));
+                    mask = mul(mask, alb);
+                    const float cosine = -dot(rd, N) / length(rd);
+                    const f3 reflected = reflect(rd, N);
+                    const f3 refracted = refract(rd, N, rough);
+                    const float prob = eq3(refracted, f3{0.0f, 0.0f, 0.0f}) ? 1.0f : schlick(cosine, rough);
+                    const float u = rng_next(salt, uvx, uvy);
+                    bool refl = u < prob;
+                    if (depth < 8u) {
+                        if (j.flags & VKR_TRACE_REPLAY_DECISIONS) refl = (drep >> (2u * depth + 1u)) & 1u;
+                        if (refl) drec |= 1u << (2u * depth + 1u);
+                    }
+                    rd = normalize(refl ? reflected : refracted);
+                    ro = P;
+                    do_rr = true;
+                } else {
+                    do_rr = true;                                    // no case in the switch
+                }
+            }
+        } else {
+            if (!tr.hit) {                                           // :475-502
+                const vkr_sphere* S = j.sc.spheres + sidx;
+                const float x = tsh / S->r + 1.0f;
+                f3 att = divs(muls(ld3(S->mat.emissive), 1.0f), x * x);
+                att = divs(subs(att, kAttSub), kAttDiv);
+                att = maxs(att, 0.0f);
+                const f3 F0 = mix3(f3{0.04f, 0.04f, 0.04f}, alb, metal);
+                const f3 V = normalize(sub(ld3(j.cam_pos), P));
+                const f3 H = normalize(add(V, Lsh));
+                const float NDF = distribution_ggx(N, H, rough);
+                const float G = geometry_smith(N, V, Lsh, rough);
+                const f3 F = fresnel_schlick(gmax(dot(H, V), 0.0f), F0);
+                f3 kD = ssub(1.0f, F);
+                kD = muls(kD, 1.0f - metal);
+                const f3 numerator = muls(F, NDF * G);
+                const float denominator = 4.0f * gmax(dot(N, V), 0.0f) * gmax(dot(N, Lsh), 0.0f);
+                const f3 specular = divs(numerator, gmax(denominator, 0.001f));
+                const float NdotL = gmax(dot(N, Lsh), 0.0f);
+                e = add(e, muls(mul(add(divs(mul(kD, alb), kPi), specular), att), NdotL));
+            }
+            ++ei;
+            if (ei >= j.sc.n_emitters) fin_diffuse = true;
+        }
+
+        if (fin_diffuse) {                                           // :509-511
+            acc = add(acc, mul(mask, add(em, e)));
+            mask = mul(mask, alb);
+            rd = normalize(add(reflect(rd, N), dj));
+            ro = P;
+            phase = kPhTrace;
+            do_rr = true;
+        }
+        if (do_rr) {                                                 // :545-549
+            const float p = max3(mask);
+            const float u = rng_next(salt, uvx, uvy);
+            bool term = u > p;
+            if (depth < 8u) {
+                if (j.flags & VKR_TRACE_REPLAY_DECISIONS) term = !((drep >> (2u * depth)) & 1u);
+                if (!term) drec |= 1u << (2u * depth);
+            }
+            if (term) {
+                path_end = true;
+            } else {
+                mask = muls(mask, 1.0f / p);
+                ++depth;
+                if (depth >= j.max_depth) path_end = true;
+                phase = kPhTrace;
+            }
+        }
+        if (path_end) {                                              // :578-581
+            if ((j.flags & VKR_TRACE_RECORD_DECISIONS) && j.decisions)
+                j.decisions[((size_t)py * j.W + px) * j.spp + smp] = (uint16_t)drec;
+            accum = add(accum, acc);
+            ++smp;
+            if (smp < j.spp) {
+                new_path = true;
+            } else {                                                 // :585-592
+                const size_t pi = (size_t)py * j.W + px;
+                f3 c = divs(accum, (float)j.spp);
+                if (j.radiance) reinterpret_cast<float4*>(j.radiance)[pi] = make_float4(c.x, c.y, c.z, 1.0f);
+                c = divv(c, adds(c, 1.0f));
+                c = f3{pow_f(c.x, kInvGamma), pow_f(c.y, kInvGamma), pow_f(c.z, kInvGamma)};
+                const float dither = rng_next(salt, uvx, uvy) / 64.0f;
+                c = adds(c, dither);
+                if (j.rgba8)
+                    reinterpret_cast<uint32_t*>(j.rgba8)[pi] =
+                        unorm8(c.x) | (unorm8(c.y) << 8) | (unorm8(c.z) << 16) | (255u << 24);
+                if (j.prim_id) j.prim_id[pi] = prim0;
+                if (j.hit_t) j.hit_t[pi] = t0;
+                if (j.ray_count) j.ray_count[pi] = rays_px;
+                has = false;
+            }
+        }
+    }
+
+    flush_counter(&A.counters->rays, c_rays);
+    flush_counter(&A.counters->shadow, c_shadow);
+    flush_counter(&A.counters->overflow, wc.overflow);
+    if (COUNT) {
+        flush_counter(&A.counters->nodes, wc.nodes);
+        flush_counter(&A.counters->tris, wc.tris);
+        flush_counter(&A.counters->spheres, wc.spheres);
+        flush_counter(&A.counters->planes, wc.planes);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Whitted kernel, Assets/Raytracer.comp (its own constant scene, :98-127).
+// ---------------------------------------------------------------------------
+namespace whitted {
+constexpr float kEps = 0.01f;  // :89
+struct Sph { uint32_t mat; float o[3]; float r; };
+struct Pln { uint32_t mat; float n[3]; float d; };
+__constant__ Sph kSph[2] = {{5, {-14.0f, 12.0f, 32.0f}, 5.0f}, {4, {32.0f, 24.0f, 25.0f}, 12.0f}};
+__constant__ Pln kPln[5] = {{0, {0.0f, 1.0f, 0.0f}, 0.0f},
+                            {0, {0.0f, -1.0f, 0.0f}, 128.0f},
+                            {2, {0.0f, 0.0f, -1.0f}, 64.0f},
+                            {1, {1.0f, 0.0f, 0.0f}, 64.0f},
+                            {3, {-1.0f, 0.0f, 0.0f}, 64.0f}};
+__constant__ float kTri[9] = {10.0f, 10.0f, 0.0f, 0.0f, 20.0f, 0.0f, -10.0f, 10.0f, 0.0f};
+__constant__ uint32_t kRefl[6] = {1, 0, 1, 0, 0, 0};
+__constant__ float kDiff[18] = {1, 1, 1, 1, 0, 0, 0, 1, 0, 0, 0, 1, 1, 1, 0, 1, 0, 1};
+
+struct Hit { uint32_t mat; float dist; f3 P, N; uint32_t prim; };
+
+__device__ __forceinline__ float wplane_t(f3 o, f3 d, f3 n, float dist)  // :180-192
+{
+    const float dd = dot(d, n);
+    if (dd == 0.0f) return 0.0f;
+    return gmax(-(dist + dot(o, n)) / dd, 0.0f);
+}
+
+__device__ bool trace(f3 o, f3 d, Hit& h)                                  // :224-278
+{
+    bool found = false;
+    const f3 v0 = ld3(kTri), v1 = ld3(kTri + 3), v2 = ld3(kTri + 6);
+    {
+        const float t = tri_t(o, d, v0, sub(v1, v0), sub(v2, v0), kEps);
+        if (t > kEps && t < h.dist) {
+            h.dist = t;
+            h.mat = 1;
+            h.P = add(o, muls(d, t));
+            const f3 u = sub(v1, v0), v = sub(v2, v0);
+            h.N = f3{(u.y * v.z) - (u.z * v.y), (u.z * v.x) - (u.x * v.z), (u.x * v.y) - (u.y * v.x)};
+            h.prim = 0;
+            found = true;
+        }
+    }
+    for (int i = 0; i < 2; ++i) {
+        const f3 c = ld3(kSph[i].o);
+        const float t = sphere_t(o, d, c, kSph[i].r);
+        if (t > kEps && t < h.dist) {
+            h.dist = t;
+            h.mat = kSph[i].mat;
+            h.P = add(o, muls(d, t));
+            h.N = divs(sub(h.P, c), kSph[i].r);
+            h.prim = (1u << 30) | (uint32_t)i;
+            found = true;
+        }
+    }
+    for (int i = 0; i < 5; ++i) {
+        const float t = wplane_t(o, d, ld3(kPln[i].n), kPln[i].d);
+        if (t > kEps && t < h.dist) {
+            h.dist = t;
+            h.mat = kPln[i].mat;
+            h.P = add(o, muls(d, t));
+            h.N = ld3(kPln[i].n);
+            h.prim = (2u << 30) | (uint32_t)i;
+            found = true;
+        }
+    }
+    return found;
+}
+
+
+struct C4 { float r, g, b, a; };
+
+__device__ C4 render(const JobDesc& j, f3& ro, f3& rd, uint32_t& bounce, uint32_t& rays, bool& cap,
+                     uint32_t& prim, float& pt)                             // :280-355
+{
+    Hit h;
+    h.dist = 1000.0f;                                                       // :88,290
+    h.prim = kMiss;
+    h.mat = 0;
+    ++rays;
+    const bool hit = trace(ro, rd, h);
+    if (cap) { cap = false; prim = hit ? h.prim : kMiss; pt = h.dist; }
+    if (!hit) return C4{0.0f, 0.0f, 0.0f, 1.0f};
+    const f3 lp = ld3(j.light_pos);
+    const f3 light_vec = normalize(sub(lp, h.P));
+    const float dist_to_light = length(sub(lp, h.P));
+    const f3 mdiff = ld3(kDiff + 3 * h.mat);
+    const float li = 540.0f / (12.56637072f * dist_to_light);               // :308
+    const f3 diffuse = muls(muls(mdiff, li), gmax(dot(h.N, light_vec), 0.0f));
+    const f3 half_vec = normalize(add(light_vec, normalize(ld3(j.cam_pos))));
+    const float spec = li * pow16(gclamp(dot(h.N, half_vec), 0.0f, 1.0f));
+    f3 color = adds(diffuse, spec);
+    Hit sh;
+    sh.dist = dist_to_light;
+    sh.prim = kMiss;
+    sh.mat = 0;
+    ++rays;
+    if (trace(h.P, light_vec, sh)) {                                        // :322-340
+        color = muls(color, 0.5f);
+        bounce = 3;
+    }
+    if (kRefl[h.mat]) {                                                     // :343-351
+        rd = sub(rd, muls(h.N, 2.0f * dot(h.N, rd)));
+        ro = h.P;
+    } else {
+        bounce = 3;
+    }
+    return C4{color.x, color.y, color.z, 1.0f};
+}
+}  // namespace whitted
+
+__global__ __launch_bounds__(256) void k_whitted(const JobDesc j, DevCounters* counters)
+{
+    const uint32_t x = blockIdx.x * 16u + (threadIdx.x & 15u);
+    const uint32_t y = blockIdx.y * 16u + (threadIdx.x >> 4);
+    uint32_t rays = 0;
+    if (x < j.W && y < j.H) {
+        const float uvx = (float)x / (float)j.W, uvy = (float)y / (float)j.H;   // :362
+        const float tx = 2.0f * uvx - 1.0f, ty = 2.0f * uvy - 1.0f;             // :366
+        f3 dir = add(add(ld3(j.cam_dir), muls(ld3(j.cam_right), tx)), muls(ld3(j.cam_up), ty));
+        dir = mul(dir, f3{j.aspect, 1.0f, j.aspect});                           // :370
+        f3 ro = ld3(j.cam_pos), rd = normalize(dir);
+        uint32_t bounce = 0, prim = kMiss;
+        bool cap = true;
+        float pt = 0.0f;
+        whitted::C4 fc = whitted::render(j, ro, rd, bounce, rays, cap, prim, pt);
+        float s = 0.4f;                                                         // :382
+        while (++bounce <= 2u) {                                                // :384-394
+            const whitted::C4 rc = whitted::render(j, ro, rd, bounce, rays, cap, prim, pt);
+            const float a = 1.0f - s;
+            const float ma = 1.0f - a;
+            const whitted::C4 m = {rc.r * ma + fc.r * a, rc.g * ma + fc.g * a, rc.b * ma + fc.b * a,
+                                   rc.a * ma + fc.a * a};
+            fc = whitted::C4{fc.r * (1.0f - s) + m.r * s, fc.g * (1.0f - s) + m.g * s,
+                             fc.b * (1.0f - s) + m.b * s, fc.a * (1.0f - s) + m.a * s};
+            s = s * 0.5f;
+        }
+        const size_t pi = (size_t)y * j.W + x;
+        if (j.radiance) reinterpret_cast<float4*>(j.radiance)[pi] = make_float4(fc.r, fc.g, fc.b, fc.a);
+        if (j.rgba8)
+            reinterpret_cast<uint32_t*>(j.rgba8)[pi] =
+                unorm8(fc.r) | (unorm8(fc.g) << 8) | (unorm8(fc.b) << 16) | (unorm8(fc.a) << 24);
+        if (j.prim_id) j.prim_id[pi] = prim;
+        if (j.hit_t) j.hit_t[pi] = pt;
+        if (j.ray_count) j.ray_count[pi] = rays;
+    }
+    flush_counter(&counters->rays, rays);
+}
+
+// ---------------------------------------------------------------------------
+// Present pass, Assets/Fullscreen.frag:14-31 (+ Fullscreen.vert:6-11).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ f3 sample_bilinear(const uint8_t* img, uint32_t w, uint32_t h, float u, float v)
+{
+    // VK_FILTER_LINEAR, CLAMP_TO_BORDER with opaque black (GraphicsDevice.cpp:770-790).
+    const float x = u * (float)w - 0.5f;
+    const float y = v * (float)h - 0.5f;
+    const float fx0 = floorf(x), fy0 = floorf(y);
+    const float ax = x - fx0, ay = y - fy0;
+    const int x0 = (int)fx0, y0 = (int)fy0;
+    f3 acc{0.0f, 0.0f, 0.0f};
+    for (int dy = 0; dy < 2; ++dy) {
+        for (int dx = 0; dx < 2; ++dx) {
+            const int xi = x0 + dx, yi = y0 + dy;
+            const float wgt = (dx ? ax : 1.0f - ax) * (dy ? ay : 1.0f - ay);
+            if (xi < 0 || yi < 0 || xi >= (int)w || yi >= (int)h) continue;
+            const uint32_t t = reinterpret_cast<const uint32_t*>(img)[(size_t)yi * w + (size_t)xi];
+            acc.x = acc.x + ((float)(t & 255u) / 255.0f) * wgt;
+            acc.y = acc.y + ((float)((t >> 8) & 255u) / 255.0f) * wgt;
+            acc.z = acc.z + ((float)((t >> 16) & 255u) / 255.0f) * wgt;
+        }
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(256) void k_filter(const uint8_t* cur, const uint8_t* prev, uint32_t sw, uint32_t sh,
+                                                uint8_t* out, uint32_t dw, uint32_t dh)
+{
+    const uint32_t x = blockIdx.x * 16u + (threadIdx.x & 15u);
+    const uint32_t y = blockIdx.y * 16u + (threadIdx.x >> 4);
+    if (x >= dw || y >= dh) return;
+    const float inv_size = 1.0f / 2048.0f;                                  // :12
+    const float u = ((float)x + 0.5f) / (float)dw;
+    const float v = 1.0f - ((float)y + 0.5f) / (float)dh;                   // :16
+    const f3 c0 = sample_bilinear(cur, sw, sh, u, v);
+    const f3 c1 = sample_bilinear(prev, sw, sh, u, v);
+    const f3 n = sample_bilinear(cur, sw, sh, u + 0.0f * inv_size, v + 1.0f * inv_size);
+    const f3 s = sample_bilinear(cur, sw, sh, u + 0.0f * inv_size, v + -1.0f * inv_size);
+    const f3 e = sample_bilinear(cur, sw, sh, u + 1.0f * inv_size, v + 0.0f * inv_size);
+    const f3 w = sample_bilinear(cur, sw, sh, u + -1.0f * inv_size, v + 0.0f * inv_size);
+    const f3 dl = sub(c0, c1);
+    const float vT = dot(dl, dl);                                           // :26
+    f3 fin;
+    if (vT > 0.0005f) fin = divs(add(add(add(n, s), e), w), 4.0f);         // :28
+    else fin = c0;
+    reinterpret_cast<uint32_t*>(out)[(size_t)y * dw + x] =
+        unorm8(fin.x) | (unorm8(fin.y) << 8) | (unorm8(fin.z) << 16) | (255u << 24);
+}
+
+// ---------------------------------------------------------------------------
+// Building-block evaluation for parity tests (op codes in vkr.h).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_debug(uint32_t op, const float* in, float* out, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const float* a = in + (size_t)i * 16;
+    float* r = out + (size_t)i * 8;
+    switch (op) {
+    case VKR_DEBUG_RAND: {  // in: cx cy salt0 -> out: 8 successive rand()
+        float salt = a[2];
+        for (int k = 0; k < 8; ++k) r[k] = rng_next(salt, a[0], a[1]);
+        break;
+    }
+    case VKR_DEBUG_TRI:     // in: o(3) d(3) v0(3) v1(3) eps ... (v2 at 12..14 via a[12]) -> t
+        r[0] = tri_t(f3{a[0], a[1], a[2]}, f3{a[3], a[4], a[5]}, f3{a[6], a[7], a[8]},
+                     sub(f3{a[9], a[10], a[11]}, f3{a[6], a[7], a[8]}),
+                     sub(f3{a[12], a[13], a[14]}, f3{a[6], a[7], a[8]}), a[15]);
+        break;
+    case VKR_DEBUG_SPHERE:  // in: o(3) d(3) c(3) r -> t
+        r[0] = sphere_t(f3{a[0], a[1], a[2]}, f3{a[3], a[4], a[5]}, f3{a[6], a[7], a[8]}, a[9]);
+        break;
+    case VKR_DEBUG_PLANE:   // in: o(3) d(3) N(3) len -> t
+        r[0] = plane_t(f3{a[0], a[1], a[2]}, f3{a[3], a[4], a[5]}, f3{a[6], a[7], a[8]}, a[9]);
+        break;
+    case VKR_DEBUG_BRDF: {  // in: N(3) H(3) V(3) L(3) rough cos F0 -> NDF, G, F(3), schlick(cos, rough)
+        const f3 Nn{a[0], a[1], a[2]}, H{a[3], a[4], a[5]}, V{a[6], a[7], a[8]}, L{a[9], a[10], a[11]};
+        r[0] = distribution_ggx(Nn, H, a[12]);
+        r[1] = geometry_smith(Nn, V, L, a[12]);
+        const f3 F = fresnel_schlick(a[13], f3{a[14], a[14], a[14]});
+        r[2] = F.x; r[3] = F.y; r[4] = F.z;
+        r[5] = schlick(a[13], a[12]);
+        break;
+    }
+    case VKR_DEBUG_JITTER: {  // in: d(3) phi sina cosa -> out(3)
+        const f3 v = jitter(f3{a[0], a[1], a[2]}, a[3], a[4], a[5]);
+        r[0] = v.x; r[1] = v.y; r[2] = v.z;
+        break;
+    }
+    case VKR_DEBUG_COS:     // in: x -> cos, sin, pow(x, 1/2.2)
+        r[0] = cos_f(a[0]);
+        r[1] = sin_f(a[0]);
+        r[2] = pow_f(a[0], kInvGamma);
+        break;
+    default:
+        break;
+    }
+}
+
+__global__ void k_zero(DevCounters* c, uint32_t* pool)
+{
+    if (threadIdx.x == 0) {
+        c->rays = c->shadow = c->nodes = c->tris = c->spheres = c->planes = c->overflow = 0ull;
+        if (pool) *pool = 0u;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+// ---------------------------------------------------------------------------
+namespace {
+template <int BLOCK, int STACK, bool COUNT, bool BATCH>
+hipError_t go(const PathLaunch& a, uint32_t grid, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_path<BLOCK, STACK, COUNT, BATCH>), dim3(grid), dim3(BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int BLOCK, int STACK>
+hipError_t go2(const PathLaunch& a, bool count, bool batch, uint32_t grid, hipStream_t s)
+{
+    if (count) return batch ? go<BLOCK, STACK, true, true>(a, grid, s) : go<BLOCK, STACK, true, false>(a, grid, s);
+    return batch ? go<BLOCK, STACK, false, true>(a, grid, s) : go<BLOCK, STACK, false, false>(a, grid, s);
+}
+
+template <int BLOCK, int STACK, bool COUNT, bool BATCH>
+const void* fn() { return reinterpret_cast<const void*>(&k_path<BLOCK, STACK, COUNT, BATCH>); }
+
+template <int BLOCK, int STACK>
+const void* fn2(bool count, bool batch)
+{
+    if (count) return batch ? fn<BLOCK, STACK, true, true>() : fn<BLOCK, STACK, true, false>();
+    return batch ? fn<BLOCK, STACK, false, true>() : fn<BLOCK, STACK, false, false>();
+}
+
+const void* kernel_ptr(uint32_t block, uint32_t stack, bool count, bool batch)
+{
+    if (block == 64) return stack <= 32 ? fn2<64, 32>(count, batch) : fn2<64, 64>(count, batch);
+    if (block == 128) return stack <= 32 ? fn2<128, 32>(count, batch) : fn2<128, 64>(count, batch);
+    return stack <= 32 ? fn2<256, 32>(count, batch) : fn2<256, 64>(count, batch);
+}
+}  // namespace
+
+uint32_t stack_variant(uint32_t needed)
+{
+    if (needed <= 32) return 32;
+    if (needed <= 64) return 64;
+    return 0;
+}
+
+hipError_t launch_path(const PathLaunch& a, uint32_t block, uint32_t stack, bool count, bool batch, uint32_t grid,
+                       hipStream_t s)
+{
+    if (block == 64) return stack <= 32 ? go2<64, 32>(a, count, batch, grid, s) : go2<64, 64>(a, count, batch, grid, s);
+    if (block == 128)
+        return stack <= 32 ? go2<128, 32>(a, count, batch, grid, s) : go2<128, 64>(a, count, batch, grid, s);
+    return stack <= 32 ? go2<256, 32>(a, count, batch, grid, s) : go2<256, 64>(a, count, batch, grid, s);
+}
+
+uint32_t persistent_grid(uint32_t block, uint32_t stack, bool count, bool batch, int device)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(block, stack, count, batch), (int)block,
+                                                     0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 256;
+    return (uint32_t)(per_cu * cus);
+}
+
+hipError_t launch_whitted(const JobDesc& j, DevCounters* c, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_whitted, dim3((j.W + 15) / 16, (j.H + 15) / 16), dim3(256), 0, s, j, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter(const uint8_t* cur, const uint8_t* prev, uint32_t sw, uint32_t sh, uint8_t* out,
+                         uint32_t dw, uint32_t dh, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_filter, dim3((dw + 15) / 16, (dh + 15) / 16), dim3(256), 0, s, cur, prev, sw, sh, out, dw,
+                       dh);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug(uint32_t op, const float* in, float* out, uint32_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_debug, dim3((n + 255) / 256), dim3(256), 0, s, op, in, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_counters(DevCounters* c, uint32_t* pool, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, c, pool);
+    return hipGetLastError();
+}
+
+}  // namespace vkr
