@@ -1,0 +1,176 @@
+"""Benchmark: Mrays/s (all bounces) of the gfx950 path kernel on one MI355X.
+
+Workload (one "step" = one frame): 1920x1080, 4 spp, max 4 bounces, 100k
+random triangles (vertices U[-50,50]^3, edges U[0.1,2]) + 256 random spheres
++ 1 emissive light sphere + a ground plane, seed 0, FP32.  Rays counted are
+every trace_ray call of the reference program (camera, bounce and
+next-event shadow rays), read from the kernel's own counters.
+
+Prints ONE JSON line.  Also reports:
+  roofline      algorithmic bytes per launch (BVH nodes, primitive records,
+                output) over the event-timed kernel duration vs 8 TB/s HBM;
+  cpu_baseline  the CPU restatement of the reference (linear scan, all host
+                threads) timed on a pixel sample of the same frame.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "vk-renderer_amd"), os.path.join(ROOT, "tests")]
+
+HBM_PEAK_GBS = 8000.0
+NODE_B, TRI_B, SPHERE_B, PLANE_B, OUT_B = 64, 48, 20, 16, 4
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--sched", type=int, default=0, help="0 auto, 1 plain, 2 persistent")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--cpu-pixels", type=int, default=64)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1",
+                          "n_gpus": a.gpus}), flush=True)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import scenes
+    import vkr
+
+    vkr.load()
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a GPU")
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+
+    wl = scenes.soup100k()
+    W, H = wl.width, wl.height
+    ctx = vkr.Context()
+    sid = ctx.upload(wl.scene)
+    info = ctx.info(sid)
+    log(f"scene: {info}")
+    rgba = torch.zeros((W * H, 4), dtype=torch.uint8, device=dev)
+    p = wl.params(sched=a.sched, block=a.block, chunk=a.chunk)
+
+    # Work-counting pass (separate kernel variant, not timed).
+    ctx.set_flags(vkr.CTX_COUNT_WORK)
+    ctx.trace(sid, wl.frame, p, rgba8=rgba)
+    work = ctx.counters()
+    ctx.set_flags(0)
+    log(f"work per frame: {work}")
+
+    for _ in range(a.warmup):
+        ctx.trace(sid, wl.frame, p, rgba8=rgba)
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.current_stream()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        evs[i][0].record(stream)
+        ctx.trace(sid, wl.frame, p, rgba8=rgba)
+        evs[i][1].record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    last_kernel_ms = ctx.last_kernel_ms()
+    cnt = ctx.counters()
+    launch_ms = [s.elapsed_time(e) for s, e in evs]
+    kern_ms = float(np.mean(launch_ms))
+    elapsed = t1 - t0
+    rays = cnt["rays"]
+    value = rays * a.steps / elapsed / 1e6
+    log(f"rays/frame {rays} shadow {cnt['shadow_rays']} overflow {cnt['window_overflow']} "
+        f"launch ms mean {kern_ms:.3f} min {min(launch_ms):.3f} last kernel (lib events) {last_kernel_ms:.3f}")
+
+    algo_bytes = (work["nodes_visited"] * NODE_B + work["tris_tested"] * TRI_B +
+                  work["spheres_tested"] * SPHERE_B + work["planes_tested"] * PLANE_B + W * H * OUT_B)
+    achieved = algo_bytes / (kern_ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_soup100k.json")
+    if os.path.exists(tf):
+        try:
+            traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+        except (ValueError, OSError):
+            traffic = None
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "algorithmic_bytes_per_launch": int(algo_bytes),
+                "kernel_ms": round(kern_ms, 4)}
+
+    variants = {}
+    if not a.no_variants:
+        for name, sched in (("plain", vkr.SCHED_PLAIN), ("persistent", vkr.SCHED_PERSISTENT)):
+            q = wl.params(sched=sched, block=a.block, chunk=a.chunk)
+            ctx.trace(sid, wl.frame, q, rgba8=rgba)
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            for _ in range(3):
+                ctx.trace(sid, wl.frame, q, rgba8=rgba)
+            torch.cuda.synchronize()
+            variants[name] = round(ctx.counters()["rays"] * 3 / (time.perf_counter() - ts) / 1e6, 2)
+        log(f"variants Mrays/s: {variants}")
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        import oracle_ffi
+        threads = min(16, os.cpu_count() or 1)
+        rng = np.random.default_rng(0)
+        pix = rng.choice(W * H, a.cpu_pixels, replace=False).astype(np.uint32)
+        ts = time.perf_counter()
+        r = oracle_ffi.trace(wl.scene, wl.frame, W, H, wl.spp, wl.max_depth, pixels=pix, threads=threads)
+        dt = time.perf_counter() - ts
+        cpu_rays = int(r["ray_count"].sum())
+        cpu = {"value": round(cpu_rays / dt / 1e6, 6), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": f"{a.cpu_pixels} random pixels of the same frame (all {wl.spp} spp x {wl.max_depth} "
+                         f"bounces, {cpu_rays} rays, {dt:.1f} s), reference linear scan over every primitive"}
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": "Mrays/s (all bounces) at 1080p 100k-tri, 1 GPU",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(elapsed / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded PCG64 scene, FrameData.seed 0)",
+        "config": {"workload": "soup100k_1920x1080_4spp_4bounce", "width": W, "height": H, "spp": wl.spp,
+                   "max_depth": wl.max_depth, "n_tris": info["n_tris"], "n_spheres": info["n_spheres"],
+                   "n_planes": info["n_planes"], "n_emitters": info["n_emitters"],
+                   "rays_per_frame": rays, "sched": a.sched, "block": a.block, "chunk": a.chunk},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "variants_mrays": variants,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()
