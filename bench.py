@@ -163,7 +163,9 @@ def main():
         "config": {"workload": "soup100k_1920x1080_4spp_4bounce", "width": W, "height": H, "spp": wl.spp,
                    "max_depth": wl.max_depth, "n_tris": info["n_tris"], "n_spheres": info["n_spheres"],
                    "n_planes": info["n_planes"], "n_emitters": info["n_emitters"],
-                   "rays_per_frame": rays, "sched": a.sched, "block": a.block, "chunk": a.chunk},
+                   "rays_per_frame": rays, "rays_traversed_per_frame": cnt["rays_traced"],
+                   "traversed_mrays": round(cnt["rays_traced"] * a.steps / elapsed / 1e6, 2),
+                   "sched": a.sched, "block": a.block, "chunk": a.chunk},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "variants_mrays": variants,

@@ -289,6 +289,9 @@ typedef struct vkr_counters {
     uint64_t spheres_tested;  /* sphere tests (VKR_CTX_COUNT_WORK only) */
     uint64_t planes_tested;   /* plane tests (VKR_CTX_COUNT_WORK only) */
     uint64_t window_overflow; /* closest-hit tie window exceeded (documented near-ties) */
+    uint64_t rays_traced;     /* rays actually traversed: shadow rays whose contribution is
+                                 exactly 0 whatever their outcome are counted in `rays`
+                                 but not traversed */
 } vkr_counters;
 
 vkr_error vkr_last_counters(vkr_ctx* ctx, vkr_counters* out);

@@ -450,6 +450,7 @@ vkr_error vkr_last_counters(vkr_ctx* ctx, vkr_counters* out)
     out->spheres_tested = c.spheres;
     out->planes_tested = c.planes;
     out->window_overflow = c.overflow;
+    out->rays_traced = c.traced;
     return VKR_SUCCESS;
 }
 

@@ -51,7 +51,7 @@ struct JobDesc {
 };
 
 struct DevCounters {
-    unsigned long long rays, shadow, nodes, tris, spheres, planes, overflow;
+    unsigned long long rays, shadow, traced, nodes, tris, spheres, planes, overflow;
 };
 
 struct PathLaunch {

@@ -265,12 +265,12 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
     float uvx = 0.0f, uvy = 0.0f, salt = 0.0f;
     uint32_t smp = 0, depth = 0, phase = kPhTrace;
     f3 ro{}, rd{}, acc{}, mask{}, accum{};
-    f3 P{}, N{}, dj{}, e{}, em{}, alb{}, Lsh{};
-    float rough = 0.0f, metal = 0.0f, tsh = 0.0f;
-    uint32_t ei = 0, sidx = 0;
+    f3 P{}, N{}, dj{}, e{}, cnee{};
+    const vkr_material* hm = nullptr;   // material of the current diffuse hit
+    uint32_t ei = 0;
     uint32_t rays_px = 0, prim0 = kMiss, drec = 0, drep = 0;
     float t0 = 0.0f;
-    uint32_t c_rays = 0, c_shadow = 0;
+    uint32_t c_rays = 0, c_shadow = 0, c_traced = 0;
     bool new_path = false;
     WorkCnt wc{0, 0, 0, 0, 0};
 
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
 
         f3 o{}, d{};
         float tmax = 0.0f;
-        bool any = false;
+        bool any = false, need = false;
         if (has) {
             if (phase == kPhTrace) {
                 acc = clamps(acc, 0.0f, 1.0f);                      // :441
@@ -366,30 +366,55 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
                 tmax = 3000.0f / (dp1 * dp1);                        // :444
                 o = ro;
                 d = rd;
+                need = true;
             } else {
-                // Next-event sample of emitter ei, :464-469.
-                sidx = j.sc.emitters[ei];
-                const vkr_sphere* S = j.sc.spheres + sidx;
+                // Next-event sample of emitter ei (:464-469) and its
+                // unoccluded contribution (:475-502), evaluated before the
+                // shadow ray: when it is exactly +-0 the ray's outcome cannot
+                // change e, so it is not traversed (it is still counted).
+                const vkr_sphere* S = j.sc.spheres + j.sc.emitters[ei];
                 const f3 sP = ld3(S->P);
                 const float sr = S->r;
-                tsh = length(sub(sP, P)) - sr;
+                const float tl = length(sub(sP, P)) - sr;
                 const f3 l0 = sub(sP, P);
                 const float cos_a_max = sqrtf(1.0f - gclamp(sr * sr / dot(l0, l0), 0.0f, 1.0f));
                 const float cosa = gmix(cos_a_max, 1.0f, rng_next(salt, uvx, uvy));
                 const float phi = kTwoPi * rng_next(salt, uvx, uvy);
-                Lsh = jitter(l0, phi, sqrtf(1.0f - cosa * cosa), cosa);
+                const f3 L = jitter(l0, phi, sqrtf(1.0f - cosa * cosa), cosa);
+
+                const float x = tl / sr + 1.0f;
+                f3 att = divs(muls(ld3(S->mat.emissive), 1.0f), x * x);
+                att = divs(subs(att, kAttSub), kAttDiv);
+                att = maxs(att, 0.0f);
+                const f3 alb = ld3(hm->albedo);
+                const float rough = hm->roughness, metal = hm->metalness;
+                const f3 F0 = mix3(f3{0.04f, 0.04f, 0.04f}, alb, metal);
+                const f3 V = normalize(sub(ld3(j.cam_pos), P));
+                const f3 H = normalize(add(V, L));
+                const float NDF = distribution_ggx(N, H, rough);
+                const float G = geometry_smith(N, V, L, rough);
+                const f3 F = fresnel_schlick(gmax(dot(H, V), 0.0f), F0);
+                f3 kD = ssub(1.0f, F);
+                kD = muls(kD, 1.0f - metal);
+                const f3 numerator = muls(F, NDF * G);
+                const float denominator = 4.0f * gmax(dot(N, V), 0.0f) * gmax(dot(N, L), 0.0f);
+                const f3 specular = divs(numerator, gmax(denominator, 0.001f));
+                const float NdotL = gmax(dot(N, L), 0.0f);
+                cnee = muls(mul(add(divs(mul(kD, alb), kPi), specular), att), NdotL);
+                need = !(cnee.x == 0.0f && cnee.y == 0.0f && cnee.z == 0.0f);
                 o = P;
-                d = Lsh;
-                tmax = tsh;
+                d = L;
+                tmax = tl;
                 any = true;
                 ++c_shadow;
             }
             ++rays_px;
             ++c_rays;
+            if (need) ++c_traced;
         }
 
         TravOut tr{kMiss, 0.0f, false};
-        if (has) tr = traverse<BLOCK, COUNT>(j.sc, o, d, tmax, j.eps, any, stk, wc);
+        if (has && need) tr = traverse<BLOCK, COUNT>(j.sc, o, d, tmax, j.eps, any, stk, wc);
         if (!has) continue;
 
         bool fin_diffuse = false, do_rr = false, path_end = false;
@@ -418,25 +443,21 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
                     N = ld3(Q->N);                                   // :424
                     m = &Q->mat;
                 }
-                alb = ld3(m->albedo);
-                const f3 emis = ld3(m->emissive);
-                rough = m->roughness;
-                metal = m->metalness;
+                hm = m;
                 const uint32_t type = m->type;
                 if (type == VKR_MAT_DIFFUSE) {                       // :451-513
                     const float r2 = rng_next(salt, uvx, uvy);
                     const float phi = kTwoPi * rng_next(salt, uvx, uvy);
-                    dj = muls(jitter(N, phi, sqrtf(r2), sqrtf(1.0f - r2)), 1.0f - metal);
+                    dj = muls(jitter(N, phi, sqrtf(r2), sqrtf(1.0f - r2)), 1.0f - m->metalness);
                     e = f3{0.0f, 0.0f, 0.0f};
-                    const bool all_pos = gmax(gsign(emis.x), 0.0f) == 1.0f && gmax(gsign(emis.y), 0.0f) == 1.0f &&
-                                         gmax(gsign(emis.z), 0.0f) == 1.0f;
-                    em = all_pos ? normalize(emis) : f3{0.0f, 0.0f, 0.0f};  // :507
                     ei = 0;
                     if (j.sc.n_emitters > 0) phase = kPhNee;
                     else fin_diffuse = true;
                 } else if (type == VKR_MAT_DIELECTRIC) {             // :514-542
+                    const f3 emis = ld3(m->emissive);
+                    const float rough = m->roughness;
                     acc = add(acc, mul(emis, mask));
-                    mask = mul(mask, alb);
+                    mask = mul(mask, ld3(m->albedo));
                     const float cosine = -dot(rd, N) / length(rd);
                     const f3 reflected = reflect(rd, N);
                     const f3 refracted = refract(rd, N, rough);
@@ -455,33 +476,18 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
                 }
             }
         } else {
-            if (!tr.hit) {                                           // :475-502
-                const vkr_sphere* S = j.sc.spheres + sidx;
-                const float x = tsh / S->r + 1.0f;
-                f3 att = divs(muls(ld3(S->mat.emissive), 1.0f), x * x);
-                att = divs(subs(att, kAttSub), kAttDiv);
-                att = maxs(att, 0.0f);
-                const f3 F0 = mix3(f3{0.04f, 0.04f, 0.04f}, alb, metal);
-                const f3 V = normalize(sub(ld3(j.cam_pos), P));
-                const f3 H = normalize(add(V, Lsh));
-                const float NDF = distribution_ggx(N, H, rough);
-                const float G = geometry_smith(N, V, Lsh, rough);
-                const f3 F = fresnel_schlick(gmax(dot(H, V), 0.0f), F0);
-                f3 kD = ssub(1.0f, F);
-                kD = muls(kD, 1.0f - metal);
-                const f3 numerator = muls(F, NDF * G);
-                const float denominator = 4.0f * gmax(dot(N, V), 0.0f) * gmax(dot(N, Lsh), 0.0f);
-                const f3 specular = divs(numerator, gmax(denominator, 0.001f));
-                const float NdotL = gmax(dot(N, Lsh), 0.0f);
-                e = add(e, muls(mul(add(divs(mul(kD, alb), kPi), specular), att), NdotL));
-            }
+            if (need && !tr.hit) e = add(e, cnee);                   // :473, :502
             ++ei;
             if (ei >= j.sc.n_emitters) fin_diffuse = true;
         }
 
-        if (fin_diffuse) {                                           // :509-511
+        if (fin_diffuse) {                                           // :507-511
+            const f3 emis = ld3(hm->emissive);
+            const bool all_pos = gmax(gsign(emis.x), 0.0f) == 1.0f && gmax(gsign(emis.y), 0.0f) == 1.0f &&
+                                 gmax(gsign(emis.z), 0.0f) == 1.0f;
+            const f3 em = all_pos ? normalize(emis) : f3{0.0f, 0.0f, 0.0f};
             acc = add(acc, mul(mask, add(em, e)));
-            mask = mul(mask, alb);
+            mask = mul(mask, ld3(hm->albedo));
             rd = normalize(add(reflect(rd, N), dj));
             ro = P;
             phase = kPhTrace;
@@ -532,6 +538,7 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
 
     flush_counter(&A.counters->rays, c_rays);
     flush_counter(&A.counters->shadow, c_shadow);
+    flush_counter(&A.counters->traced, c_traced);
     flush_counter(&A.counters->overflow, wc.overflow);
     if (COUNT) {
         flush_counter(&A.counters->nodes, wc.nodes);
@@ -687,6 +694,7 @@ __global__ __launch_bounds__(256) void k_whitted(const JobDesc j, DevCounters* c
         if (j.ray_count) j.ray_count[pi] = rays;
     }
     flush_counter(&counters->rays, rays);
+    flush_counter(&counters->traced, rays);
 }
 
 // ---------------------------------------------------------------------------
@@ -792,7 +800,7 @@ __global__ __launch_bounds__(256) void k_debug(uint32_t op, const float* in, flo
 __global__ void k_zero(DevCounters* c, uint32_t* pool)
 {
     if (threadIdx.x == 0) {
-        c->rays = c->shadow = c->nodes = c->tris = c->spheres = c->planes = c->overflow = 0ull;
+        c->rays = c->shadow = c->traced = c->nodes = c->tris = c->spheres = c->planes = c->overflow = 0ull;
         if (pool) *pool = 0u;
     }
 }

@@ -7,9 +7,9 @@
 //   length = correctly rounded sqrt(dot(v,v)); normalize = v / length(v)
 //   min/max/clamp: IEEE minNum/maxNum (a NaN operand yields the other one)
 //   sign(NaN) = 0
-//   cos/sin: the float nearest to the double-precision cos/sin
+//   cos/sin: the float nearest to a double-precision cos/sin
 //   pow(x,5), pow(x,16): products in double rounded once; pow(x,2) = x*x;
-//   other exponents through double pow; pow(x<0, y) = NaN
+//   other exponents via double exp(y * log(x)); pow(x<0, y) = NaN
 //   constants: folded in double and rounded once to float
 // All translation units are compiled with -ffp-contract=off and correctly
 // rounded float division and square root.  Fused multiply-adds appear only
@@ -69,8 +69,142 @@ __device__ __forceinline__ f3 refract(f3 I, f3 N, float eta)
 }
 __device__ __forceinline__ float max3(f3 e) { return gmax(gmax(e.x, e.y), e.z); }
 
-__device__ __forceinline__ float cos_f(float x) { return (float)::cos((double)x); }
-__device__ __forceinline__ float sin_f(float x) { return (float)::sin((double)x); }
+// --------------------------------------------------------------------------
+// Transcendentals.  Each returns the float nearest to a double-precision
+// evaluation accurate to a few double ulps, which equals the correctly
+// rounded float result except when the exact value lies within ~1e-15
+// (relative) of a float rounding boundary.  The methods are fdlibm's
+// published ones (Sun Microsystems, freely distributable): Cody-Waite
+// argument reduction by pi/2 with the product n*pi/2 carried as exact
+// double-double terms (accurate for |x| < 2^32), then the k_sin/k_cos
+// minimax polynomials; log via s = f/(2+f) and the Lg1..Lg7 series; exp via
+// reduction by ln2 and the P1..P5 rational form.  They are written here to
+// keep register use low; OCML's general versions cost ~50 VGPRs more.
+// --------------------------------------------------------------------------
+namespace detail {
+constexpr double kPio2_1 = 1.57079632679489655800e+00;   // RN(pi/2)
+constexpr double kPio2_2 = 6.12323399573676603587e-17;   // RN(pi/2 - kPio2_1)
+constexpr double kPio2_3 = -1.4973849048591698e-33;      // RN(pi/2 - kPio2_1 - kPio2_2)
+constexpr double kTwoOverPi = 6.36619772367581382433e-01;
+
+__device__ __forceinline__ double poly_sin(double x)
+{
+    const double S1 = -1.66666666666666324348e-01, S2 = 8.33333333332248946124e-03,
+                 S3 = -1.98412698298579493134e-04, S4 = 2.75573137070700676789e-06,
+                 S5 = -2.50507602534068634195e-08, S6 = 1.58969099521155010221e-10;
+    const double z = x * x;
+    const double v = z * x;
+    const double r = S2 + z * (S3 + z * (S4 + z * (S5 + z * S6)));
+    return x + v * (S1 + z * r);
+}
+
+__device__ __forceinline__ double poly_cos(double x)
+{
+    const double C1 = 4.16666666666666019037e-02, C2 = -1.38888888888741095749e-03,
+                 C3 = 2.48015872894767294178e-05, C4 = -2.75573143513906633035e-07,
+                 C5 = 2.08757232129817482790e-09, C6 = -1.13596475577881948265e-11;
+    const double z = x * x;
+    const double r = z * (C1 + z * (C2 + z * (C3 + z * (C4 + z * (C5 + z * C6)))));
+    const double ax = fabs(x);
+    if (ax < 0.3) return 1.0 - (0.5 * z - z * r);
+    // |x| >= 0.3: split 1 - z/2 so the subtraction stays exact (fdlibm).
+    const double qx = ax > 0.78125 ? 0.28125 : __hiloint2double(__double2hiint(ax) - 0x00200000, 0);
+    const double hz = 0.5 * z - qx;
+    const double a = 1.0 - qx;
+    return a - (hz - z * r);
+}
+
+// x - n*pi/2 for the nearest integer n; returns r and n mod 4.
+__device__ __forceinline__ double reduce_pio2(double x, int& quadrant)
+{
+    const double n = rint(x * kTwoOverPi);
+    const double p_hi = n * kPio2_1;
+    const double p_lo = __builtin_fma(n, kPio2_1, -p_hi);       // exact remainder of n * P1
+    const double q_hi = n * kPio2_2;
+    const double q_lo = __builtin_fma(n, kPio2_2, -q_hi);
+    double r = x - p_hi;                                         // exact (Sterbenz)
+    r = r - p_lo;
+    r = r - q_hi;
+    r = r - (q_lo + n * kPio2_3);
+    quadrant = (int)((long long)n & 3);   // n mod 4, also for negative n
+    return r;
+}
+
+__device__ __forceinline__ void sincos_d(double x, double& s, double& c)
+{
+    int q;
+    const double r = reduce_pio2(x, q);
+    const double ps = poly_sin(r), pc = poly_cos(r);
+    switch (q) {
+    case 0: s = ps; c = pc; break;
+    case 1: s = pc; c = -ps; break;
+    case 2: s = -ps; c = -pc; break;
+    default: s = -pc; c = ps; break;
+    }
+}
+
+__device__ __forceinline__ double log_d(double x)  // x > 0, finite
+{
+    const double Lg1 = 6.666666666666735130e-01, Lg2 = 3.999999999940941908e-01,
+                 Lg3 = 2.857142874366239149e-01, Lg4 = 2.222219843214978396e-01,
+                 Lg5 = 1.818357216161805012e-01, Lg6 = 1.531383769920937332e-01,
+                 Lg7 = 1.479819860511658591e-01;
+    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+    int k;
+    double m = frexp(x, &k);        // x = m * 2^k, m in [0.5, 1)
+    if (m < 0.70710678118654752440) {
+        m = m * 2.0;
+        k -= 1;
+    }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    const double w = z * z;
+    const double t1 = w * (Lg2 + w * (Lg4 + w * Lg6));
+    const double t2 = z * (Lg1 + w * (Lg3 + w * (Lg5 + w * Lg7)));
+    const double R = t2 + t1;
+    const double hfsq = 0.5 * f * f;
+    const double dk = (double)k;
+    return dk * ln2_hi - ((hfsq - (s * (hfsq + R) + dk * ln2_lo)) - f);
+}
+
+__device__ __forceinline__ double exp_d(double t)  // finite t, |t| < 700
+{
+    const double P1 = 1.66666666666666019037e-01, P2 = -2.77777777770155933842e-03,
+                 P3 = 6.61375632143793436117e-05, P4 = -1.65339022054652515390e-06,
+                 P5 = 4.13813679705723846039e-08;
+    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+    const double inv_ln2 = 1.44269504088896338700e+00;
+    const double k = rint(t * inv_ln2);
+    const double hi = t - k * ln2_hi;
+    const double lo = k * ln2_lo;
+    const double r = hi - lo;
+    const double r2 = r * r;
+    const double c = r - r2 * (P1 + r2 * (P2 + r2 * (P3 + r2 * (P4 + r2 * P5))));
+    const double y = 1.0 - ((lo - (r * c) / (2.0 - c)) - hi);
+    return ldexp(y, (int)k);
+}
+}  // namespace detail
+
+__device__ __forceinline__ float cos_f(float x)
+{
+    double s, c;
+    detail::sincos_d((double)x, s, c);
+    return (float)c;
+}
+__device__ __forceinline__ float sin_f(float x)
+{
+    double s, c;
+    detail::sincos_d((double)x, s, c);
+    return (float)s;
+}
+__device__ __forceinline__ void sincos_f(float x, float& s, float& c)
+{
+    double sd, cd;
+    detail::sincos_d((double)x, sd, cd);
+    s = (float)sd;
+    c = (float)cd;
+}
 __device__ __forceinline__ float pow5(float x)
 {
     if (x < 0.0f) return __builtin_nanf("");
@@ -87,10 +221,14 @@ __device__ __forceinline__ float pow16(float x)
     const double d8 = d4 * d4;
     return (float)(d8 * d8);
 }
+// pow(x, y) for the tone curve (x in [0, 1], y = 1/2.2).
 __device__ __forceinline__ float pow_f(float x, float y)
 {
     if (x < 0.0f) return __builtin_nanf("");
-    return (float)::pow((double)x, (double)y);
+    if (x != x) return x;
+    if (x == 0.0f) return y > 0.0f ? 0.0f : (y == 0.0f ? 1.0f : __builtin_huge_valf());
+    if (x == __builtin_huge_valf()) return y > 0.0f ? x : (y == 0.0f ? 1.0f : 0.0f);
+    return (float)detail::exp_d((double)y * detail::log_d((double)x));
 }
 
 // GLSL constants of Assets/Tracer.comp, folded in double then rounded.
@@ -120,7 +258,9 @@ __device__ __forceinline__ f3 jitter(f3 d, float phi, float sina, float cosa)
     const f3 w = normalize(d);
     const f3 u = normalize(cross(f3{w.y, w.z, w.x}, w));
     const f3 v = cross(w, u);
-    return add(muls(add(muls(u, cos_f(phi)), muls(v, sin_f(phi))), sina), muls(w, cosa));
+    float sp, cp;
+    sincos_f(phi, sp, cp);
+    return add(muls(add(muls(u, cp), muls(v, sp)), sina), muls(w, cosa));
 }
 
 // Assets/Tracer.comp:265-312.

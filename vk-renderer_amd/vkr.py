@@ -98,7 +98,7 @@ class Outputs(C.Structure):
 class Counters(C.Structure):
     _fields_ = [("rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("nodes_visited", C.c_uint64),
                 ("tris_tested", C.c_uint64), ("spheres_tested", C.c_uint64), ("planes_tested", C.c_uint64),
-                ("window_overflow", C.c_uint64)]
+                ("window_overflow", C.c_uint64), ("rays_traced", C.c_uint64)]
 
 
 class FrameData(C.Structure):
