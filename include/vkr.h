@@ -269,6 +269,24 @@ vkr_error vkr_batch_graph_create(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32
 vkr_error vkr_graph_launch(vkr_ctx* ctx, uint32_t graph, void* stream);
 vkr_error vkr_graph_destroy(vkr_ctx* ctx, uint32_t graph);
 
+/* Build extension: the project autotuner.  Times the path kernel on this
+ * scene and frame for every compiled (workgroup size, stack depth,
+ * pool-chunk, schedule) candidate, `repeats` timed runs each after one warm
+ * run, and remembers the fastest for later calls on the same scene and image
+ * size that leave block / chunk at 0 and sched at VKR_SCHED_AUTO.  outputs
+ * are the (device) targets the timing runs write.  Synchronous. */
+typedef struct vkr_tune_result {
+    uint32_t block, chunk, stack, sched; /* the winner */
+    float    best_ms;
+    uint32_t n_tried;
+    uint32_t cand_block[64], cand_chunk[64], cand_stack[64], cand_sched[64];
+    float    cand_ms[64];
+} vkr_tune_result;
+
+vkr_error vkr_autotune(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* frame_data,
+                       const vkr_trace_params* params, const vkr_outputs* outputs, uint32_t repeats,
+                       vkr_tune_result* out);
+
 /* Build extension: path-replay adjoint of the path kernel.  For loss gradient
  * d_radiance (device, W*H*4 floats, dLoss/d(mean radiance rgb); a ignored)
  * accumulates dLoss/dparams into d_params (device, 7 * n_spheres + 1 floats,

@@ -17,179 +17,14 @@
 
 #include "../../include/vkr.h"
 #include "bvh.hpp"
+#include "ctx.hpp"
 #include "kernels.hpp"
 
 using namespace vkr;
 
-namespace {
-
+namespace vkr {
 thread_local std::string g_err = "no error";
-
-vkr_error fail(vkr_error code, const std::string& msg)
-{
-    g_err = msg;
-    return code;
-}
-
-vkr_error fail_hip(const char* what, hipError_t e)
-{
-    g_err = std::string(what) + ": " + hipGetErrorString(e);
-    return e == hipErrorOutOfMemory ? VKR_OUT_OF_MEMORY : VKR_HIP_ERROR;
-}
-
-#define VKR_HIP(call)                                    \
-    do {                                                 \
-        const hipError_t e_ = (call);                    \
-        if (e_ != hipSuccess) return fail_hip(#call, e_); \
-    } while (0)
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { reset(); }
-    void reset()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    hipError_t alloc(size_t n)
-    {
-        reset();
-        if (n == 0) n = 16;
-        const hipError_t e = hipMalloc(&p, n);
-        if (e == hipSuccess) bytes = n;
-        else p = nullptr;
-        return e;
-    }
-    hipError_t upload(const void* src, size_t n)
-    {
-        const hipError_t e = alloc(n);
-        if (e != hipSuccess) return e;
-        if (n && src) return hipMemcpy(p, src, n, hipMemcpyHostToDevice);
-        return hipSuccess;
-    }
-    template <class T> T* as() const { return static_cast<T*>(p); }
-};
-
-const vkr_material kMirror = {{1.0f, 0.5f, 0.5f}, 0.0f, {0.0f, 0.0f, 0.0f}, 0.0f, 1.0f, VKR_MAT_DIFFUSE, {0.0f, 0.0f}};
-
-struct Scene {
-    DevBuf nodes, tris, sph, sph_key, tris_orig, spheres, planes, tri_mat, materials, emitters;
-    std::vector<vkr_sphere> host_spheres;
-    DevScene dev{};
-    vkr_scene_info info{};
-};
-
-struct Graph {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    DevBuf jobs, job_start, pool, counters;
-    uint64_t total = 0;
-    ~Graph()
-    {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-    }
-};
-
-}  // namespace
-
-struct vkr_ctx {
-    int device = 0;
-    uint32_t flags = 0;
-    std::vector<std::unique_ptr<Scene>> scenes;
-    std::vector<std::unique_ptr<Graph>> graphs;
-    DevBuf counters, pool, jobs, job_start;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    bool timed = false;
-    DevCounters* last_counters = nullptr;
-};
-
-namespace {
-
-Scene* get_scene(vkr_ctx* ctx, uint32_t id)
-{
-    if (!ctx || id >= ctx->scenes.size()) return nullptr;
-    return ctx->scenes[id].get();
-}
-
-bool nonzero3(const float* v) { return !(v[0] == 0.0f && v[1] == 0.0f && v[2] == 0.0f); }
-
-std::vector<uint32_t> emitter_list(const std::vector<vkr_sphere>& s)
-{
-    // Tracer.comp:458-462: every sphere whose emissive is not vec3(0), in order.
-    std::vector<uint32_t> e;
-    for (uint32_t i = 0; i < s.size(); ++i)
-        if (nonzero3(s[i].mat.emissive)) e.push_back(i);
-    return e;
-}
-
-vkr_error validate_params(const vkr_trace_params* p)
-{
-    if (!p) return fail(VKR_INVALID_ARGUMENT, "params is NULL");
-    if (p->width == 0 || p->height == 0 || p->width > 16384 || p->height > 16384)
-        return fail(VKR_INVALID_ARGUMENT, "image size must be 1..16384 per side");
-    if (p->kernel != VKR_KERNEL_PATH && p->kernel != VKR_KERNEL_WHITTED)
-        return fail(VKR_INVALID_ARGUMENT, "unknown kernel");
-    if (p->kernel == VKR_KERNEL_PATH) {
-        if (p->spp == 0 || p->spp > 4096) return fail(VKR_INVALID_ARGUMENT, "spp must be 1..4096");
-        if (p->max_depth == 0 || p->max_depth > 16) return fail(VKR_INVALID_ARGUMENT, "max_depth must be 1..16");
-        if (!(p->epsilon >= 0.0f) || !std::isfinite(p->epsilon))
-            return fail(VKR_INVALID_ARGUMENT, "epsilon must be finite and >= 0");
-    }
-    if (p->block != 0 && p->block != 64 && p->block != 128 && p->block != 256)
-        return fail(VKR_INVALID_ARGUMENT, "block must be 0, 64, 128 or 256");
-    if (p->chunk > 64) return fail(VKR_INVALID_ARGUMENT, "chunk must be 0..64");
-    if ((p->flags & VKR_TRACE_RECORD_DECISIONS) && (p->flags & VKR_TRACE_REPLAY_DECISIONS))
-        return fail(VKR_INVALID_ARGUMENT, "record and replay decisions are exclusive");
-    return VKR_SUCCESS;
-}
-
-JobDesc make_job(const Scene& sc, const vkr_frame_data& fd, const vkr_trace_params& p, const vkr_outputs* out)
-{
-    JobDesc j{};
-    j.sc = sc.dev;
-    for (int k = 0; k < 3; ++k) {
-        j.cam_pos[k] = fd.camera.pos[k];
-        j.cam_dir[k] = fd.camera.dir[k];
-        j.cam_right[k] = fd.camera.right[k];
-        j.cam_up[k] = fd.camera.up[k];
-        j.light_pos[k] = fd.light_pos[k];
-    }
-    j.aspect = fd.aspect_ratio;
-    j.seed = fd.seed;
-    j.eps = p.epsilon == 0.0f ? 1e-3f : p.epsilon;
-    j.flags = p.flags;
-    j.kernel = p.kernel;
-    j.W = p.width;
-    j.H = p.height;
-    j.spp = p.spp;
-    j.max_depth = p.max_depth;
-    j.tiles_x = (p.width + 7u) / 8u;
-    j.n_tiles = j.tiles_x * ((p.height + 7u) / 8u);
-    if (out) {
-        j.rgba8 = out->rgba8;
-        j.radiance = out->radiance;
-        j.prim_id = out->prim_id;
-        j.hit_t = out->hit_t;
-        j.ray_count = out->ray_count;
-        j.decisions = out->decisions;
-    }
-    return j;
-}
-
-vkr_error ensure_events(vkr_ctx* ctx)
-{
-    if (!ctx->ev0) VKR_HIP(hipEventCreate(&ctx->ev0));
-    if (!ctx->ev1) VKR_HIP(hipEventCreate(&ctx->ev1));
-    return VKR_SUCCESS;
-}
-
-}  // namespace
+}  // namespace vkr
 
 extern "C" {
 
@@ -228,6 +63,8 @@ vkr_error vkr_destroy(vkr_ctx* ctx)
     (void)hipDeviceSynchronize();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->ev_copy) (void)hipEventDestroy(ctx->ev_copy);
+    if (ctx->host_jobs) (void)hipHostFree(ctx->host_jobs);
     delete ctx;
     return VKR_SUCCESS;
 }
@@ -419,12 +256,19 @@ vkr_error vkr_trace_frame(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd
         a.total = j.n_tiles * 64u;
         a.pool = pool;
         a.counters = cnt;
-        const uint32_t block = params->block ? params->block : 128u;
-        const bool persistent = params->sched != VKR_SCHED_PLAIN;
+        uint32_t block = params->block, chunk = params->chunk, sched = params->sched;
+        uint32_t stack = s->dev.stack_needed;
+        if (const TuneEntry* t = find_tuned(ctx, scene, *params)) {
+            if (!block) block = t->block;
+            if (!chunk) chunk = t->chunk;
+            if (sched == VKR_SCHED_AUTO) sched = t->sched;
+            stack = std::max(stack, t->stack);
+        }
+        if (!block) block = 128u;
+        const bool persistent = sched != VKR_SCHED_PLAIN;
         a.persistent = persistent ? 1u : 0u;
-        a.chunk = params->chunk ? params->chunk : 32u;
+        a.chunk = chunk ? chunk : 32u;
         const bool count = (ctx->flags & VKR_CTX_COUNT_WORK) != 0;
-        const uint32_t stack = s->dev.stack_needed;
         const uint32_t grid = persistent ? std::min(persistent_grid(block, stack, count, false, ctx->device),
                                                     (a.total + block - 1u) / block)
                                          : (a.total + block - 1u) / block;
@@ -432,6 +276,7 @@ vkr_error vkr_trace_frame(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd
     }
     VKR_HIP(hipEventRecord(ctx->ev1, st));
     ctx->timed = true;
+    ctx->last_counters = nullptr;
     return VKR_SUCCESS;
 }
 
@@ -517,19 +362,6 @@ vkr_error vkr_camera_from_angles(const float pos[3], float pitch, float yaw, vkr
     out->up[0] = up.x; out->up[1] = up.y; out->up[2] = up.z;
     return VKR_SUCCESS;
 }
-
-vkr_error vkr_trace_batch(vkr_ctx*, const vkr_scene_job*, uint32_t, uint32_t, uint32_t, void*)
-{
-    return fail(VKR_UNSUPPORTED, "batched tracing is not built yet");
-}
-
-vkr_error vkr_batch_graph_create(vkr_ctx*, const vkr_scene_job*, uint32_t, uint32_t, uint32_t, void*, uint32_t*)
-{
-    return fail(VKR_UNSUPPORTED, "batched tracing is not built yet");
-}
-
-vkr_error vkr_graph_launch(vkr_ctx*, uint32_t, void*) { return fail(VKR_UNSUPPORTED, "no graphs"); }
-vkr_error vkr_graph_destroy(vkr_ctx*, uint32_t) { return fail(VKR_UNSUPPORTED, "no graphs"); }
 
 vkr_error vkr_adjoint(vkr_ctx*, uint32_t, const vkr_frame_data*, const vkr_trace_params*, const float*, float*,
                       void*)

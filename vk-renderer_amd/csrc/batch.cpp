@@ -1,2 +1,280 @@
-// batch.cpp - batched persistent launches and hipGraph capture (see vkr.h).
+// batch.cpp - batched persistent launches, hipGraph capture and the
+// autotuner (build extensions of the reference's per-frame Draw,
+// Source/GraphicsDevice.cpp:1215-1342, which dispatches one image at a time).
+//
+// A batch is one persistent launch of the BATCH variant of k_path: its waves
+// pull 8x8-tile work items of every job from one global pool, so small images
+// and cheap scenes fill the machine together and no wave idles on a launch
+// tail.  Each job's results are bit-identical to vkr_trace_frame.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "../../include/vkr.h"
+#include "ctx.hpp"
 #include "kernels.hpp"
+
+using namespace vkr;
+
+namespace {
+
+struct BatchPlan {
+    std::vector<JobDesc> jobs;
+    std::vector<uint32_t> start;  // n + 1 pool offsets
+    uint32_t stack = 32;
+    uint64_t total = 0;
+};
+
+vkr_error plan_batch(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n, BatchPlan& plan)
+{
+    if (!jobs || n == 0) return fail(VKR_INVALID_ARGUMENT, "empty batch");
+    plan.jobs.resize(n);
+    plan.start.resize(n + 1);
+    uint64_t off = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const vkr_scene_job& jb = jobs[i];
+        const vkr_error pe = validate_params(&jb.params);
+        if (pe != VKR_SUCCESS) return pe;
+        if (jb.params.kernel != VKR_KERNEL_PATH) return fail(VKR_INVALID_ARGUMENT, "batch jobs must use the path kernel");
+        if ((jb.params.flags & (VKR_TRACE_RECORD_DECISIONS | VKR_TRACE_REPLAY_DECISIONS)) &&
+            (!jb.outputs.decisions || jb.params.max_depth > 8))
+            return fail(VKR_INVALID_ARGUMENT, "decision flags need outputs.decisions and max_depth <= 8");
+        Scene* s = get_scene(ctx, jb.scene);
+        if (!s) return fail(VKR_INVALID_ARGUMENT, "unknown scene in batch job");
+        plan.jobs[i] = make_job(*s, jb.frame_data, jb.params, &jb.outputs);
+        plan.stack = std::max(plan.stack, s->dev.stack_needed);
+        plan.start[i] = (uint32_t)off;
+        off += (uint64_t)plan.jobs[i].n_tiles * 64u;
+        if (off >= 0xFFFFFFF0ull) return fail(VKR_INVALID_ARGUMENT, "batch exceeds 2^32 pool entries");
+    }
+    plan.start[n] = (uint32_t)off;
+    plan.total = off;
+    return VKR_SUCCESS;
+}
+
+PathLaunch batch_launch(const BatchPlan& plan, const JobDesc* djobs, const uint32_t* dstart, uint32_t n,
+                        uint32_t chunk, uint32_t* pool, DevCounters* cnt)
+{
+    PathLaunch a{};
+    a.jobs = djobs;
+    a.job_start = dstart;
+    a.n_jobs = n;
+    a.total = (uint32_t)plan.total;
+    a.persistent = 1u;
+    a.chunk = chunk ? chunk : 32u;
+    a.pool = pool;
+    a.counters = cnt;
+    return a;
+}
+
+uint32_t batch_grid(const BatchPlan& plan, uint32_t block, bool count, int device)
+{
+    const uint32_t need = (uint32_t)((plan.total + block - 1u) / block);
+    return std::max(1u, std::min(persistent_grid(block, plan.stack, count, true, device), need));
+}
+
+bool valid_block(uint32_t b) { return b == 0 || b == 64 || b == 128 || b == 256; }
+
+}  // namespace
+
+extern "C" {
+
+vkr_error vkr_trace_batch(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n_jobs, uint32_t block, uint32_t chunk,
+                          void* stream)
+{
+    if (!ctx) return fail(VKR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!valid_block(block) || chunk > 64) return fail(VKR_INVALID_ARGUMENT, "block must be 0/64/128/256, chunk <= 64");
+    BatchPlan plan;
+    const vkr_error pe = plan_batch(ctx, jobs, n_jobs, plan);
+    if (pe != VKR_SUCCESS) return pe;
+    VKR_HIP(hipSetDevice(ctx->device));
+    const vkr_error ee = ensure_events(ctx);
+    if (ee != VKR_SUCCESS) return ee;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+
+    // Job table: staged through pinned memory, copied in stream order after
+    // the previous launch of this context finished with the device copy.
+    const size_t jb = plan.jobs.size() * sizeof(JobDesc);
+    const size_t sb = plan.start.size() * sizeof(uint32_t);
+    VKR_HIP(hipEventSynchronize(ctx->ev_copy));
+    if (ctx->host_jobs_bytes < jb + sb) {
+        if (ctx->host_jobs) VKR_HIP(hipHostFree(ctx->host_jobs));
+        ctx->host_jobs = nullptr;
+        ctx->host_jobs_bytes = 0;
+        VKR_HIP(hipHostMalloc(&ctx->host_jobs, jb + sb, hipHostMallocDefault));
+        ctx->host_jobs_bytes = jb + sb;
+    }
+    if (ctx->jobs.bytes < jb) VKR_HIP(ctx->jobs.alloc(jb));
+    if (ctx->job_start.bytes < sb) VKR_HIP(ctx->job_start.alloc(sb));
+    std::memcpy(ctx->host_jobs, plan.jobs.data(), jb);
+    std::memcpy(static_cast<char*>(ctx->host_jobs) + jb, plan.start.data(), sb);
+    if (ctx->timed) VKR_HIP(hipStreamWaitEvent(st, ctx->ev1, 0));
+    VKR_HIP(hipMemcpyAsync(ctx->jobs.p, ctx->host_jobs, jb, hipMemcpyHostToDevice, st));
+    VKR_HIP(hipMemcpyAsync(ctx->job_start.p, static_cast<char*>(ctx->host_jobs) + jb, sb, hipMemcpyHostToDevice, st));
+    VKR_HIP(hipEventRecord(ctx->ev_copy, st));
+
+    DevCounters* cnt = ctx->counters.as<DevCounters>();
+    uint32_t* pool = ctx->pool.as<uint32_t>();
+    const uint32_t blk = block ? block : 128u;
+    const bool count = (ctx->flags & VKR_CTX_COUNT_WORK) != 0;
+    const PathLaunch a = batch_launch(plan, ctx->jobs.as<JobDesc>(), ctx->job_start.as<uint32_t>(), n_jobs, chunk,
+                                      pool, cnt);
+    VKR_HIP(launch_zero_counters(cnt, pool, st));
+    VKR_HIP(hipEventRecord(ctx->ev0, st));
+    VKR_HIP(launch_path(a, blk, plan.stack, count, true, batch_grid(plan, blk, count, ctx->device), st));
+    VKR_HIP(hipEventRecord(ctx->ev1, st));
+    ctx->timed = true;
+    ctx->last_counters = nullptr;
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_batch_graph_create(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n_jobs, uint32_t block,
+                                 uint32_t chunk, void* stream, uint32_t* out_graph)
+{
+    if (!ctx || !out_graph) return fail(VKR_INVALID_ARGUMENT, "NULL argument");
+    if (!valid_block(block) || chunk > 64) return fail(VKR_INVALID_ARGUMENT, "block must be 0/64/128/256, chunk <= 64");
+    BatchPlan plan;
+    const vkr_error pe = plan_batch(ctx, jobs, n_jobs, plan);
+    if (pe != VKR_SUCCESS) return pe;
+    VKR_HIP(hipSetDevice(ctx->device));
+    auto g = std::make_unique<Graph>();
+    VKR_HIP(g->jobs.upload(plan.jobs.data(), plan.jobs.size() * sizeof(JobDesc)));
+    VKR_HIP(g->job_start.upload(plan.start.data(), plan.start.size() * sizeof(uint32_t)));
+    VKR_HIP(g->pool.alloc(64));
+    VKR_HIP(g->counters.alloc(sizeof(DevCounters)));
+    g->total = plan.total;
+
+    // Capture on a private stream: a fill kernel resets the pool cursor and the
+    // counters (no memset nodes), then the persistent batch kernel.
+    hipStream_t cap = nullptr;
+    VKR_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    const uint32_t blk = block ? block : 128u;
+    const bool count = (ctx->flags & VKR_CTX_COUNT_WORK) != 0;
+    const PathLaunch a = batch_launch(plan, g->jobs.as<JobDesc>(), g->job_start.as<uint32_t>(), n_jobs, chunk,
+                                      g->pool.as<uint32_t>(), g->counters.as<DevCounters>());
+    const uint32_t grid = batch_grid(plan, blk, count, ctx->device);
+    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        const hipError_t e1 = launch_zero_counters(g->counters.as<DevCounters>(), g->pool.as<uint32_t>(), cap);
+        const hipError_t e2 = launch_path(a, blk, plan.stack, count, true, grid, cap);
+        e = hipStreamEndCapture(cap, &g->graph);
+        if (e == hipSuccess && e1 != hipSuccess) e = e1;
+        if (e == hipSuccess && e2 != hipSuccess) e = e2;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    (void)hipStreamDestroy(cap);
+    if (e != hipSuccess) return fail_hip("batch graph capture", e);
+    (void)stream;
+    for (uint32_t i = 0; i < ctx->graphs.size(); ++i) {
+        if (!ctx->graphs[i]) {
+            ctx->graphs[i] = std::move(g);
+            *out_graph = i;
+            return VKR_SUCCESS;
+        }
+    }
+    ctx->graphs.push_back(std::move(g));
+    *out_graph = (uint32_t)ctx->graphs.size() - 1u;
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_graph_launch(vkr_ctx* ctx, uint32_t graph, void* stream)
+{
+    if (!ctx || graph >= ctx->graphs.size() || !ctx->graphs[graph]) return fail(VKR_INVALID_ARGUMENT, "unknown graph");
+    VKR_HIP(hipSetDevice(ctx->device));
+    const vkr_error ee = ensure_events(ctx);
+    if (ee != VKR_SUCCESS) return ee;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    Graph& g = *ctx->graphs[graph];
+    VKR_HIP(hipEventRecord(ctx->ev0, st));
+    VKR_HIP(hipGraphLaunch(g.exec, st));
+    VKR_HIP(hipEventRecord(ctx->ev1, st));
+    ctx->timed = true;
+    ctx->last_counters = g.counters.as<DevCounters>();
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_graph_destroy(vkr_ctx* ctx, uint32_t graph)
+{
+    if (!ctx || graph >= ctx->graphs.size() || !ctx->graphs[graph]) return fail(VKR_INVALID_ARGUMENT, "unknown graph");
+    VKR_HIP(hipSetDevice(ctx->device));
+    VKR_HIP(hipDeviceSynchronize());
+    if (ctx->last_counters == ctx->graphs[graph]->counters.as<DevCounters>()) ctx->last_counters = nullptr;
+    ctx->graphs[graph].reset();
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_autotune(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd, const vkr_trace_params* params,
+                       const vkr_outputs* outputs, uint32_t repeats, vkr_tune_result* out)
+{
+    if (!ctx || !fd || !params || !outputs || !out) return fail(VKR_INVALID_ARGUMENT, "NULL argument");
+    Scene* s = get_scene(ctx, scene);
+    if (!s) return fail(VKR_INVALID_ARGUMENT, "unknown scene");
+    if (params->kernel != VKR_KERNEL_PATH) return fail(VKR_INVALID_ARGUMENT, "the autotuner tunes the path kernel");
+    const vkr_error pe = validate_params(params);
+    if (pe != VKR_SUCCESS) return pe;
+    VKR_HIP(hipSetDevice(ctx->device));
+    if (repeats == 0) repeats = 1;
+    std::memset(out, 0, sizeof(*out));
+
+    struct Cand { uint32_t block, chunk, stack, sched; };
+    std::vector<Cand> cands;
+    const uint32_t stacks[2] = {32u, 64u};
+    for (uint32_t b : {64u, 128u, 256u}) cands.push_back({b, 0u, s->dev.stack_needed, (uint32_t)VKR_SCHED_PLAIN});
+    for (uint32_t st : stacks) {
+        if (st < s->dev.stack_needed) continue;
+        for (uint32_t b : {64u, 128u, 256u})
+            for (uint32_t c : {1u, 8u, 32u, 64u}) cands.push_back({b, c, st, (uint32_t)VKR_SCHED_PERSISTENT});
+    }
+    // Drop any previous tuning of this (scene, size) so candidates run as given.
+    ctx->tuned.erase(std::remove_if(ctx->tuned.begin(), ctx->tuned.end(),
+                                    [&](const TuneEntry& t) {
+                                        return t.scene == scene && t.width == params->width &&
+                                               t.height == params->height;
+                                    }),
+                     ctx->tuned.end());
+    float best = 3.4e38f;
+    Cand win = cands[0];
+    for (const Cand& c : cands) {
+        vkr_trace_params p = *params;
+        p.block = c.block;
+        p.chunk = c.chunk;
+        p.sched = c.sched;
+        // The stack variant is chosen through a temporary tuning entry.
+        ctx->tuned.push_back(TuneEntry{scene, p.width, p.height, c.block, c.chunk ? c.chunk : 32u, c.stack, c.sched});
+        float total = 0.0f;
+        vkr_error e = vkr_trace_frame(ctx, scene, fd, &p, outputs, nullptr);
+        for (uint32_t r = 0; r < repeats && e == VKR_SUCCESS; ++r) {
+            e = vkr_trace_frame(ctx, scene, fd, &p, outputs, nullptr);
+            float ms = 0.0f;
+            if (e == VKR_SUCCESS) e = vkr_last_kernel_ms(ctx, &ms);
+            total += ms;
+        }
+        ctx->tuned.pop_back();
+        if (e != VKR_SUCCESS) return e;
+        const float ms = total / (float)repeats;
+        if (out->n_tried < 64) {
+            const uint32_t k = out->n_tried++;
+            out->cand_block[k] = c.block;
+            out->cand_chunk[k] = c.chunk;
+            out->cand_stack[k] = c.stack;
+            out->cand_sched[k] = c.sched;
+            out->cand_ms[k] = ms;
+        }
+        if (ms < best) {
+            best = ms;
+            win = c;
+        }
+    }
+    ctx->tuned.push_back(TuneEntry{scene, params->width, params->height, win.block, win.chunk ? win.chunk : 32u,
+                                   win.stack, win.sched});
+    out->block = win.block;
+    out->chunk = win.chunk;
+    out->stack = win.stack;
+    out->sched = win.sched;
+    out->best_ms = best;
+    return VKR_SUCCESS;
+}
+
+}  // extern "C"

@@ -859,6 +859,15 @@ hipError_t launch_path(const PathLaunch& a, uint32_t block, uint32_t stack, bool
 
 uint32_t persistent_grid(uint32_t block, uint32_t stack, bool count, bool batch, int device)
 {
+    // Resident workgroups per CU x CUs, cached per (device, variant).
+    struct Entry { int device; uint32_t block, stack; bool count, batch; uint32_t grid; };
+    static thread_local Entry cache[32];
+    static thread_local int n = 0;
+    for (int i = 0; i < n; ++i) {
+        const Entry& c = cache[i];
+        if (c.device == device && c.block == block && c.stack == stack && c.count == count && c.batch == batch)
+            return c.grid;
+    }
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(block, stack, count, batch), (int)block,
                                                      0) != hipSuccess ||
@@ -866,7 +875,9 @@ uint32_t persistent_grid(uint32_t block, uint32_t stack, bool count, bool batch,
         per_cu = 1;
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 256;
-    return (uint32_t)(per_cu * cus);
+    const uint32_t grid = (uint32_t)(per_cu * cus);
+    if (n < 32) cache[n++] = Entry{device, block, stack, count, batch, grid};
+    return grid;
 }
 
 hipError_t launch_whitted(const JobDesc& j, DevCounters* c, hipStream_t s)

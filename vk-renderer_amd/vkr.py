@@ -113,6 +113,13 @@ class CameraC(C.Structure):
     _fields_ = [("raw", C.c_float * 16)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("block", C.c_uint32), ("chunk", C.c_uint32), ("stack", C.c_uint32), ("sched", C.c_uint32),
+                ("best_ms", C.c_float), ("n_tried", C.c_uint32),
+                ("cand_block", C.c_uint32 * 64), ("cand_chunk", C.c_uint32 * 64), ("cand_stack", C.c_uint32 * 64),
+                ("cand_sched", C.c_uint32 * 64), ("cand_ms", C.c_float * 64)]
+
+
 _lib = None
 
 
@@ -154,6 +161,7 @@ def load(path: str = LIB_PATH):
         "vkr_filter_frame": (e, [vp, vp, vp, u32, u32, vp, u32, u32, vp]),
         "vkr_camera_from_angles": (e, [C.POINTER(C.c_float), C.c_float, C.c_float, C.POINTER(CameraC)]),
         "vkr_debug_eval": (e, [vp, u32, vp, vp, u32, vp]),
+        "vkr_autotune": (e, [vp, u32, vp, C.POINTER(TraceParams), C.POINTER(Outputs), u32, C.POINTER(TuneResult)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -328,6 +336,18 @@ class Context:
 
     def graph_destroy(self, gid: int):
         check(self.lib.vkr_graph_destroy(self.h, gid), "vkr_graph_destroy")
+
+    def autotune(self, sid: int, frame: np.ndarray, params: TraceParams, repeats: int = 1, rgba8=None, radiance=None):
+        """Runs the library autotuner; returns the winner and the timing table."""
+        fr = np.ascontiguousarray(np.asarray(frame, FRAME).reshape(1))
+        out = Outputs(_tptr(rgba8), _tptr(radiance), None, None, None, None)
+        r = TuneResult()
+        check(self.lib.vkr_autotune(self.h, sid, fr.ctypes.data, C.byref(params), C.byref(out), repeats, C.byref(r)),
+              "vkr_autotune")
+        table = [dict(block=r.cand_block[i], chunk=r.cand_chunk[i], stack=r.cand_stack[i], sched=r.cand_sched[i],
+                      ms=round(float(r.cand_ms[i]), 4)) for i in range(r.n_tried)]
+        return dict(block=r.block, chunk=r.chunk, stack=r.stack, sched=r.sched, best_ms=float(r.best_ms),
+                    table=table)
 
     def adjoint(self, sid: int, frame: np.ndarray, params: TraceParams, d_radiance, d_params, stream=None):
         fr = np.ascontiguousarray(np.asarray(frame, FRAME).reshape(1))
