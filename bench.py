@@ -38,11 +38,75 @@ def parse():
     ap.add_argument("--cpu-pixels", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--workload", default="soup100k", choices=["soup100k", "batch64"])
+    ap.add_argument("--scenes", type=int, default=64, help="batch64: number of scenes")
     return ap.parse_args()
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def bench_batch(a):
+    """64 scenes (1e3..1e6 triangles log-uniform, 256^2..1080p, 1..8 spp, 4
+    bounces) traced three ways: one plain launch per scene, one persistent
+    launch per scene, and ONE batched persistent launch (eager and replayed
+    from a captured hipGraph).  value = batched Mrays/s."""
+    import torch
+
+    import scenes
+    import vkr
+
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    wls = scenes.batch_jobs(a.scenes)
+    ctx = vkr.Context()
+    sids = [ctx.upload(wl.scene) for wl in wls]
+    log(f"built {len(wls)} scenes in {time.perf_counter() - t0:.1f} s; "
+        f"tris {sum(len(w.scene.tris) for w in wls)}, pixels*spp {sum(w.width * w.height * w.spp for w in wls)}")
+    outs = [torch.zeros((wl.width * wl.height, 4), dtype=torch.uint8, device=dev) for wl in wls]
+
+    def per_scene(sched):
+        for wl, sid, o in zip(wls, sids, outs):
+            ctx.trace(sid, wl.frame, wl.params(sched=sched), rgba8=o)
+
+    jobs = [ctx.job(sid, wl.frame, wl.params(), rgba8=o) for wl, sid, o in zip(wls, sids, outs)]
+    ctx.trace_batch(jobs, block=a.block, chunk=a.chunk)
+    torch.cuda.synchronize()
+    rays = ctx.counters()["rays"]
+    traced = ctx.counters()["rays_traced"]
+    gid = ctx.batch_graph(jobs, block=a.block, chunk=a.chunk)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        for _ in range(a.steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - ts) / a.steps
+
+    res = {}
+    res["plain_per_scene"] = timed(lambda: per_scene(vkr.SCHED_PLAIN))
+    res["persistent_per_scene"] = timed(lambda: per_scene(vkr.SCHED_PERSISTENT))
+    res["batched"] = timed(lambda: ctx.trace_batch(jobs, block=a.block, chunk=a.chunk))
+    res["batched_graph"] = timed(lambda: ctx.graph_launch(gid))
+    mr = {k: round(rays / v / 1e6, 2) for k, v in res.items()}
+    log(f"seconds per batch {res}\nMrays/s {mr}")
+    out = {
+        "metric": "Mrays/s (all bounces), 64-scene batched persistent launch, 1 GPU",
+        "value": mr["batched_graph"], "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps, "warmup": 1,
+        "ms_per_step": round(res["batched_graph"] * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded PCG64 scenes)",
+        "config": {"workload": f"batch{len(wls)}", "rays_per_batch": rays, "rays_traversed_per_batch": traced,
+                   "block": a.block, "chunk": a.chunk},
+        "variants_mrays": mr,
+        "speedup_batched_vs_plain_per_scene": round(res["plain_per_scene"] / res["batched_graph"], 3),
+        "speedup_batched_vs_persistent_per_scene": round(res["persistent_per_scene"] / res["batched_graph"], 3),
+    }
+    print(json.dumps(out), flush=True)
+    ctx.graph_destroy(gid)
+    ctx.close()
 
 
 def main():
@@ -51,6 +115,12 @@ def main():
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1",
                           "n_gpus": a.gpus}), flush=True)
         sys.exit(2)
+    if a.workload == "batch64":
+        import torch
+        if not torch.cuda.is_available():
+            raise SystemExit("bench.py needs a GPU")
+        bench_batch(a)
+        return
 
     import numpy as np
     import torch

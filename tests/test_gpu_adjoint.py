@@ -1,0 +1,115 @@
+"""GPU: the path-replay adjoint against central finite differences with
+common random numbers and frozen branch decisions (h = 1e-3)."""
+import numpy as np
+import pytest
+
+import scenes
+import vkr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def ctx(torch_cuda):
+    c = vkr.Context()
+    yield c
+    c.close()
+
+
+def _forward(ctx, torch, sid, wl, theta, flags=0, decisions=None):
+    ctx.set_sphere_params(sid, theta)
+    n = wl.width * wl.height
+    rad = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+    ctx.trace(sid, wl.frame, wl.params(flags=flags), radiance=rad, decisions=decisions)
+    torch.cuda.synchronize()
+    return rad[:, :3].double().cpu().numpy()
+
+
+def _setup(ctx, torch, w=128, spp=16):
+    wl = scenes.adjoint_scene(width=w, height=w, spp=spp)
+    sid = ctx.upload(wl.scene)
+    theta = scenes.sphere_params(wl.scene.spheres, light_scale=1.0)
+    dec = torch.zeros(w * w * spp, dtype=torch.int16, device="cuda")
+    base = _forward(ctx, torch, sid, wl, theta, vkr.TRACE_RECORD_DECISIONS, dec)
+    return wl, sid, theta, dec, base
+
+
+def _fd(ctx, torch, sid, wl, theta, dec, i, weights):
+    h = 1e-3 * max(1.0, abs(float(theta[i])))
+    tp, tm = theta.copy(), theta.copy()
+    tp[i] += h
+    tm[i] -= h
+    lp = _forward(ctx, torch, sid, wl, tp, vkr.TRACE_REPLAY_DECISIONS, dec)
+    lm = _forward(ctx, torch, sid, wl, tm, vkr.TRACE_REPLAY_DECISIONS, dec)
+    return float(np.sum(weights * (lp - lm)) / (float(tp[i]) - float(tm[i])))
+
+
+def test_adjoint_matches_finite_differences(ctx, torch_cuda):
+    torch = torch_cuda
+    import diff
+    wl, sid, theta, dec, base = _setup(ctx, torch)
+    n_sph = len(wl.scene.spheres)
+    rng = np.random.default_rng(0)
+    target = _forward(ctx, torch, sid, wl, (theta * (1 + rng.uniform(-0.2, 0.2, theta.size))).astype(np.float32))
+    # L2 loss against the perturbed-parameter target: dLoss/dL = 2 (L - L*)
+    t = torch.tensor(theta, device="cuda", requires_grad=True)
+    rad = diff.render(t, ctx, sid, wl.frame, wl.params())
+    loss = ((rad.double() - torch.from_numpy(target).cuda()) ** 2).sum()
+    loss.backward()
+    g = t.grad.double().cpu().numpy()
+    weights = 2.0 * (base - target)
+    # Check every parameter that can move the image: albedo and roughness of
+    # all spheres, the light's emission and the light scale.
+    idx = [7 * s + k for s in range(n_sph) for k in (0, 1, 2, 6)]
+    idx += [7 * (n_sph - 1) + k for k in (3, 4, 5)] + [7 * n_sph]
+    fd = np.array([_fd(ctx, torch, sid, wl, theta, dec, i, weights) for i in idx])
+    ga = g[idx]
+    scale = np.abs(fd).max()
+    rel = np.abs(ga - fd) / np.maximum(np.abs(fd), 1e-3 * scale)
+    worst = np.argsort(-rel)[:8]
+    print("\n[adjoint] max|fd| %.4g  median rel %.3g  max rel %.3g" % (scale, np.median(rel), rel.max()))
+    for k in worst:
+        print(f"  param {idx[k]:4d} adjoint {ga[k]: .6e} fd {fd[k]: .6e} rel {rel[k]:.3g}")
+    big = np.abs(fd) > 1e-2 * scale
+    assert big.sum() >= 20
+    assert np.all(rel[big] <= 1e-3), rel[big].max()
+    # zero-emission spheres: the structural zero (no emitter, :462)
+    assert np.all(g[[7 * s + 3 for s in range(n_sph - 1)]] == 0.0)
+
+
+def test_adjoint_cost(ctx, torch_cuda):
+    torch = torch_cuda
+    import time
+    wl = scenes.adjoint_scene()
+    sid = ctx.upload(wl.scene)
+    n = wl.width * wl.height
+    rad = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+    g4 = torch.ones((n, 4), dtype=torch.float32, device="cuda")
+    d = torch.zeros(7 * len(wl.scene.spheres) + 1, dtype=torch.float32, device="cuda")
+
+    def fwd():
+        ctx.trace(sid, wl.frame, wl.params(), radiance=rad)
+
+    def adj():
+        ctx.adjoint(sid, wl.frame, wl.params(), g4, d)
+
+    res = {}
+    for name, fn in (("forward", fwd), ("adjoint", adj)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            fn()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0) / 5
+    ratio = res["adjoint"] / res["forward"]
+    print(f"\n[adjoint cost] forward {res['forward'] * 1e3:.2f} ms adjoint {res['adjoint'] * 1e3:.2f} ms ratio {ratio:.2f}")
+    assert ratio <= 3.0

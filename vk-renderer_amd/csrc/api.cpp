@@ -115,6 +115,18 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
 
     sc->host_spheres.assign(desc->spheres, desc->spheres + desc->n_spheres);
     const std::vector<uint32_t> em = emitter_list(sc->host_spheres);
+    // Parameters as uploaded: light scale 1, emission = the sphere's emissive.
+    sc->host_theta.assign(7u * desc->n_spheres + 1u, 0.0f);
+    for (uint32_t i = 0; i < desc->n_spheres; ++i) {
+        const vkr_material& m = desc->spheres[i].mat;
+        for (int k = 0; k < 3; ++k) {
+            sc->host_theta[7u * i + k] = m.albedo[k];
+            sc->host_theta[7u * i + 3u + k] = m.emissive[k];
+        }
+        sc->host_theta[7u * i + 6u] = m.roughness;
+    }
+    sc->host_theta[7u * desc->n_spheres] = 1.0f;
+    VKR_HIP(sc->theta.upload(sc->host_theta.data(), sc->host_theta.size() * sizeof(float)));
 
     VKR_HIP(sc->nodes.upload(bo.nodes.data(), bo.nodes.size() * sizeof(BvhNode)));
     VKR_HIP(sc->tris.upload(bo.tris.data(), bo.tris.size() * sizeof(TriRecord)));
@@ -220,6 +232,8 @@ vkr_error vkr_scene_set_sphere_params(vkr_ctx* ctx, uint32_t scene, const float*
     }
     s->dev.n_emitters = (uint32_t)em.size();
     s->info.n_emitters = (uint32_t)em.size();
+    s->host_theta.assign(params, params + n);
+    VKR_HIP(hipMemcpy(s->theta.p, s->host_theta.data(), (size_t)n * sizeof(float), hipMemcpyHostToDevice));
     return VKR_SUCCESS;
 }
 
@@ -363,10 +377,37 @@ vkr_error vkr_camera_from_angles(const float pos[3], float pitch, float yaw, vkr
     return VKR_SUCCESS;
 }
 
-vkr_error vkr_adjoint(vkr_ctx*, uint32_t, const vkr_frame_data*, const vkr_trace_params*, const float*, float*,
-                      void*)
+vkr_error vkr_adjoint(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd, const vkr_trace_params* params,
+                      const float* d_radiance, float* d_params, void* stream)
 {
-    return fail(VKR_UNSUPPORTED, "adjoint is not built yet");
+    if (!ctx || !fd || !d_radiance || !d_params) return fail(VKR_INVALID_ARGUMENT, "NULL argument");
+    const vkr_error pe = validate_params(params);
+    if (pe != VKR_SUCCESS) return pe;
+    if (params->kernel != VKR_KERNEL_PATH) return fail(VKR_INVALID_ARGUMENT, "the adjoint differentiates the path kernel");
+    if (params->max_depth > 10) return fail(VKR_INVALID_ARGUMENT, "the adjoint supports max_depth <= 10");
+    if (params->flags & (VKR_TRACE_RECORD_DECISIONS | VKR_TRACE_REPLAY_DECISIONS))
+        return fail(VKR_INVALID_ARGUMENT, "the adjoint replays the natural decisions; no decision flags");
+    Scene* s = get_scene(ctx, scene);
+    if (!s) return fail(VKR_INVALID_ARGUMENT, "unknown scene");
+    VKR_HIP(hipSetDevice(ctx->device));
+    const vkr_error ee = ensure_events(ctx);
+    if (ee != VKR_SUCCESS) return ee;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    AdjointLaunch a{};
+    a.job = make_job(*s, *fd, *params, nullptr);
+    a.d_radiance = d_radiance;
+    a.d_params = d_params;
+    a.params = s->theta.as<float>();
+    a.counters = ctx->counters.as<DevCounters>();
+    a.pool = ctx->pool.as<uint32_t>();
+    a.total = a.job.n_tiles * 64u;
+    VKR_HIP(launch_zero_counters(a.counters, a.pool, st));
+    VKR_HIP(hipEventRecord(ctx->ev0, st));
+    VKR_HIP(launch_adjoint(a, s->dev.stack_needed, st));
+    VKR_HIP(hipEventRecord(ctx->ev1, st));
+    ctx->timed = true;
+    ctx->last_counters = nullptr;
+    return VKR_SUCCESS;
 }
 
 }  // extern "C"

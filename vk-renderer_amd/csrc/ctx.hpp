@@ -73,7 +73,9 @@ const vkr_material kMirror = {{1.0f, 0.5f, 0.5f}, 0.0f, {0.0f, 0.0f, 0.0f}, 0.0f
 
 struct Scene {
     DevBuf nodes, tris, sph, sph_key, tris_orig, spheres, planes, tri_mat, materials, emitters;
+    DevBuf theta;                    // device copy of the sphere parameters (adjoint)
     std::vector<vkr_sphere> host_spheres;
+    std::vector<float> host_theta;   // 7 * n_spheres + 1
     DevScene dev{};
     vkr_scene_info info{};
 };

@@ -73,6 +73,8 @@ struct AdjointLaunch {
     float* d_params;              // 7 * n_spheres + 1
     const float* params;          // 7 * n_spheres + 1 (current values)
     DevCounters* counters;
+    uint32_t* pool;               // persistent pool cursor
+    uint32_t total;               // pool indices (64 per 8x8 tile)
 };
 
 hipError_t launch_path(const PathLaunch& a, uint32_t block, uint32_t stack, bool count, bool batch,

@@ -20,232 +20,10 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
+#include "traverse.hpp"
 #include "vkr_device.hpp"
 
 namespace vkr {
-
-constexpr uint32_t kDone = 0xFFFFFFFFu;
-constexpr uint32_t kPhTrace = 0;
-constexpr uint32_t kPhNee = 1;
-constexpr int kCand = 4;
-
-struct WorkCnt {
-    uint32_t nodes, tris, spheres, planes, overflow;
-};
-
-struct TravOut {
-    uint32_t prim;  // (kind << 30) | index, kMiss when nothing was accepted
-    float t;        // accepted t (t_max when missed)
-    bool hit;       // closest: something accepted; any: occluded
-};
-
-__device__ __forceinline__ void cswap(float& ta, uint32_t& ka, float& tb, uint32_t& kb)
-{
-    const bool s = kb < ka;
-    const float t0 = s ? tb : ta, t1 = s ? ta : tb;
-    const uint32_t k0 = s ? kb : ka, k1 = s ? ka : kb;
-    ta = t0; tb = t1; ka = k0; kb = k1;
-}
-
-// trace_ray of Assets/Tracer.comp:374-431 over the BVH.
-//   any == false: closest hit with the reference's acceptance rule.
-//   any == true : only "was anything accepted" (shadow rays, :473).
-template <int BLOCK, bool COUNT>
-__device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, float tmax, float eps, bool any,
-                                            uint32_t* stk, WorkCnt& wc)
-{
-    const float tmax_eps = tmax + eps;
-    const float win = 4.0f * eps;
-    const float inf = __builtin_huge_valf();
-    float ct0 = inf, ct1 = inf, ct2 = inf, ct3 = inf;
-    uint32_t ck0 = 0, ck1 = 0, ck2 = 0, ck3 = 0;
-    float tmin = inf;
-    bool ovf = false;
-    bool occluded = false;
-    float cull = any ? tmax_eps : tmax_eps + win;
-
-    const bool nan_ray = (d.x != d.x) || (d.y != d.y) || (d.z != d.z) || (o.x != o.x) || (o.y != o.y) || (o.z != o.z);
-    const BvhNode* __restrict__ nodes = sc.nodes;
-    const TriRecord* __restrict__ tris = sc.tris;
-    const float4* __restrict__ sph = sc.sph;
-    const uint32_t* __restrict__ sph_key = sc.sph_key;
-
-    if (sc.has_bvh && !nan_ray) {
-        const float ix = 1.0f / (fabsf(d.x) > 1e-20f ? d.x : copysignf(1e-20f, d.x));
-        const float iy = 1.0f / (fabsf(d.y) > 1e-20f ? d.y : copysignf(1e-20f, d.y));
-        const float iz = 1.0f / (fabsf(d.z) > 1e-20f ? d.z : copysignf(1e-20f, d.z));
-        const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
-        uint32_t ref = 0;
-        int sp = 0;
-        while (ref != kDone) {
-            while (!(ref & kRefLeaf)) {
-                const float4* np = reinterpret_cast<const float4*>(nodes + ref);
-                const float4 A = np[0], B = np[1], C = np[2];
-                const uint4 D = reinterpret_cast<const uint4*>(np)[3];
-                if (COUNT) wc.nodes++;
-                const float cb = cull + fabsf(cull) * 1e-5f + 1e-5f;
-                const float a0 = __builtin_fmaf(A.x, ix, -ox), a1 = __builtin_fmaf(A.y, ix, -ox);
-                const float a2 = __builtin_fmaf(A.z, iy, -oy), a3 = __builtin_fmaf(A.w, iy, -oy);
-                const float a4 = __builtin_fmaf(B.x, iz, -oz), a5 = __builtin_fmaf(B.y, iz, -oz);
-                const float b0 = __builtin_fmaf(B.z, ix, -ox), b1 = __builtin_fmaf(B.w, ix, -ox);
-                const float b2 = __builtin_fmaf(C.x, iy, -oy), b3 = __builtin_fmaf(C.y, iy, -oy);
-                const float b4 = __builtin_fmaf(C.z, iz, -oz), b5 = __builtin_fmaf(C.w, iz, -oz);
-                const float n0 = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), 0.0f));
-                const float f0 = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), cb));
-                const float n1 = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), 0.0f));
-                const float f1 = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), cb));
-                const bool h0 = n0 <= f0, h1 = n1 <= f1;
-                if (h0 && h1) {
-                    const bool first0 = n0 <= n1;
-                    const uint32_t nearr = first0 ? D.x : D.y;
-                    const uint32_t farr = first0 ? D.y : D.x;
-                    stk[sp * BLOCK] = farr;
-                    ++sp;
-                    ref = nearr;
-                } else if (h0) {
-                    ref = D.x;
-                } else if (h1) {
-                    ref = D.y;
-                } else {
-                    ref = sp > 0 ? stk[(sp - 1) * BLOCK] : kDone;
-                    sp = sp > 0 ? sp - 1 : 0;
-                }
-            }
-            if (ref == kDone) break;
-
-            // Leaf: test its primitives.
-            const uint32_t count = ((ref >> kRefCountShift) & 7u) + 1u;
-            const uint32_t first = ref & kRefFirstMask;
-            for (uint32_t i = 0; i < count; ++i) {
-                float t;
-                uint32_t key;
-                if (ref & kRefSphere) {
-                    const float4 s4 = sph[first + i];
-                    if (COUNT) wc.spheres++;
-                    t = sphere_t(o, d, f3{s4.x, s4.y, s4.z}, s4.w);
-                    key = 0x80000000u | sph_key[first + i];
-                } else {
-                    const float4* tp = reinterpret_cast<const float4*>(tris + first + i);
-                    const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
-                    if (COUNT) wc.tris++;
-                    t = tri_t(o, d, f3{p0.x, p0.y, p0.z}, f3{p1.x, p1.y, p1.z}, f3{p2.x, p2.y, p2.z}, eps);
-                    key = __float_as_uint(p0.w);
-                }
-                if (!(t > eps)) continue;
-                if (any) {
-                    if (t < tmax_eps) occluded = true;
-                } else if (t < fminf(tmin, tmax_eps) + win) {
-                    if (t < tmin) {
-                        tmin = t;
-                        const float lim = tmin + win;
-                        ct0 = ct0 < lim ? ct0 : inf;
-                        ct1 = ct1 < lim ? ct1 : inf;
-                        ct2 = ct2 < lim ? ct2 : inf;
-                        ct3 = ct3 < lim ? ct3 : inf;
-                    }
-                    if (ct0 == inf) { ct0 = t; ck0 = key; }
-                    else if (ct1 == inf) { ct1 = t; ck1 = key; }
-                    else if (ct2 == inf) { ct2 = t; ck2 = key; }
-                    else if (ct3 == inf) { ct3 = t; ck3 = key; }
-                    else {
-                        // List full: keep the four smallest t (documented near-tie).
-                        ovf = true;
-                        float mx = ct0; int mi = 0;
-                        if (ct1 > mx) { mx = ct1; mi = 1; }
-                        if (ct2 > mx) { mx = ct2; mi = 2; }
-                        if (ct3 > mx) { mx = ct3; mi = 3; }
-                        if (t < mx) {
-                            if (mi == 0) { ct0 = t; ck0 = key; }
-                            else if (mi == 1) { ct1 = t; ck1 = key; }
-                            else if (mi == 2) { ct2 = t; ck2 = key; }
-                            else { ct3 = t; ck3 = key; }
-                        }
-                    }
-                    cull = fminf(tmin, tmax_eps) + win;
-                }
-            }
-            if (occluded) break;
-            ref = sp > 0 ? stk[(sp - 1) * BLOCK] : kDone;
-            sp = sp > 0 ? sp - 1 : 0;
-        }
-    }
-
-    TravOut r;
-    const uint32_t np = sc.n_planes;
-    const vkr_plane* __restrict__ planes = sc.planes;
-    if (any) {
-        if (!occluded && !nan_ray) {
-            const float lim = tmax - eps;
-            for (uint32_t i = 0; i < np; ++i) {
-                if (COUNT) wc.planes++;
-                const float t = plane_t(o, d, ld3(planes[i].N), planes[i].len);
-                if ((t > eps) && (t < lim)) { occluded = true; break; }
-            }
-        }
-        r.hit = occluded;
-        r.prim = kMiss;
-        r.t = tmax;
-        return r;
-    }
-    // Replay the scan's sequential rule over the candidates in scan order.
-    cswap(ct0, ck0, ct1, ck1);
-    cswap(ct2, ck2, ct3, ck3);
-    cswap(ct0, ck0, ct2, ck2);
-    cswap(ct1, ck1, ct3, ck3);
-    cswap(ct1, ck1, ct2, ck2);
-    float best = tmax;
-    uint32_t win_key = kMiss;
-    const float top = tmin + win - eps;
-#define VKR_REPLAY(T, K)                          \
-    if ((T) < best + eps) {                       \
-        best = (T);                               \
-        win_key = (K);                            \
-        if (best >= top) ovf = true;              \
-    }
-    VKR_REPLAY(ct0, ck0)
-    VKR_REPLAY(ct1, ck1)
-    VKR_REPLAY(ct2, ck2)
-    VKR_REPLAY(ct3, ck3)
-#undef VKR_REPLAY
-    uint32_t prim = kMiss;
-    if (win_key != kMiss) prim = (win_key & 0x80000000u) ? ((1u << 30) | (win_key & 0x7FFFFFFFu)) : win_key;
-    if (!nan_ray) {
-        for (uint32_t i = 0; i < np; ++i) {
-            if (COUNT) wc.planes++;
-            const float t = plane_t(o, d, ld3(planes[i].N), planes[i].len);
-            if ((t > eps) && (t < best - eps)) {
-                best = t;
-                prim = (2u << 30) | i;
-            }
-        }
-    }
-    if (ovf) wc.overflow++;
-    r.hit = prim != kMiss;
-    r.prim = prim;
-    r.t = best;
-    return r;
-}
-
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
-{
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-__device__ __forceinline__ void flush_counter(unsigned long long* dst, uint32_t v)
-{
-    const uint32_t s = wave_sum(v);
-    if (__lane_id() == 0 && s) atomicAdd(dst, (unsigned long long)s);
-}
-
-// XCD-aware remap: consecutive workgroup ids land on different XCDs, so give
-// each XCD a contiguous range of tiles (its L2 then sees neighbouring rays).
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t g)
-{
-    const uint32_t q = g / 8u, r = g % 8u;
-    const uint32_t x = b % 8u, i = b / 8u;
-    return (x < r) ? x * (q + 1u) + i : r * (q + 1u) + (x - r) * q + i;
-}
 
 // The path program of Assets/Tracer.comp:433-593 as a per-lane state machine.
 // Each loop iteration: (1) refill idle lanes, (2) set up this lane's ray
