@@ -33,7 +33,7 @@ def _forward(ctx, torch, sid, wl, theta, flags=0, decisions=None):
     return rad[:, :3].double().cpu().numpy()
 
 
-def _setup(ctx, torch, w=128, spp=16):
+def _setup(ctx, torch, w=256, spp=16):
     wl = scenes.adjoint_scene(width=w, height=w, spp=spp)
     sid = ctx.upload(wl.scene)
     theta = scenes.sphere_params(wl.scene.spheres, light_scale=1.0)
@@ -66,9 +66,12 @@ def test_adjoint_matches_finite_differences(ctx, torch_cuda):
     loss.backward()
     g = t.grad.double().cpu().numpy()
     weights = 2.0 * (base - target)
-    # Check every parameter that can move the image: albedo and roughness of
-    # all spheres, the light's emission and the light scale.
-    idx = [7 * s + k for s in range(n_sph) for k in (0, 1, 2, 6)]
+    # Every parameter that can move the image: albedo and roughness of the 32
+    # spheres, the light's emission and the light scale.  The light's own
+    # albedo (1,1,1) is left out: with mask (1,1,1) the max3 of Russian
+    # roulette (:545) has an exact three-way tie there, a kink where a
+    # central difference averages two one-sided slopes.
+    idx = [7 * s + k for s in range(n_sph - 1) for k in (0, 1, 2, 6)]
     idx += [7 * (n_sph - 1) + k for k in (3, 4, 5)] + [7 * n_sph]
     fd = np.array([_fd(ctx, torch, sid, wl, theta, dec, i, weights) for i in idx])
     ga = g[idx]
@@ -78,9 +81,11 @@ def test_adjoint_matches_finite_differences(ctx, torch_cuda):
     print("\n[adjoint] max|fd| %.4g  median rel %.3g  max rel %.3g" % (scale, np.median(rel), rel.max()))
     for k in worst:
         print(f"  param {idx[k]:4d} adjoint {ga[k]: .6e} fd {fd[k]: .6e} rel {rel[k]:.3g}")
-    big = np.abs(fd) > 1e-2 * scale
-    assert big.sum() >= 20
+    big = np.abs(fd) > 1e-3 * scale
+    assert big.sum() >= 40
     assert np.all(rel[big] <= 1e-3), rel[big].max()
+    # aggregate relative error of the gradient vector
+    assert np.linalg.norm(ga - fd) <= 1e-3 * np.linalg.norm(fd)
     # zero-emission spheres: the structural zero (no emitter, :462)
     assert np.all(g[[7 * s + 3 for s in range(n_sph - 1)]] == 0.0)
 

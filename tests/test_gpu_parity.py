@@ -221,6 +221,25 @@ def test_soup_pixel_subset(ctx, torch_cuda):
     assert st["prim_id"] == 1.0 and st["radiance_1e-4"] >= 0.99
 
 
+def test_heightfield_pixel_subset(ctx, torch_cuda):
+    """2.09M-triangle fBm heightfield + 4096 metal/dielectric spheres, 8 spp,
+    8 bounces at 1920x1080: the oracle's linear scan on a pixel sample."""
+    wl = scenes.heightfield()
+    sid = ctx.upload(wl.scene)
+    info = ctx.info(sid)
+    print(f"\n[heightfield] bvh {info}", flush=True)
+    g = gpu_trace(ctx, sid, wl)
+    cnt = ctx.counters()
+    rng = np.random.default_rng(11)
+    W, H = wl.width, wl.height
+    pix = rng.choice(W * H, 24, replace=False).astype(np.uint32)
+    r = oracle_ffi.trace(wl.scene, wl.frame, W, H, wl.spp, wl.max_depth, pixels=pix)
+    st = compare(g, r, pixels=pix)
+    _report("heightfield_subset", st)
+    print(f"[heightfield] counters {cnt}", flush=True)
+    assert st["prim_id"] == 1.0 and st["radiance_1e-4"] >= 0.95
+
+
 def test_schedules_identical(ctx, torch_cuda):
     """Plain and persistent launches, every block size: identical bits."""
     wl = scenes.reference_scene(256, 192, spp=2, max_depth=4)
