@@ -165,6 +165,7 @@ typedef struct vkr_scene_desc {
     uint32_t flags;                  /* VKR_SCENE_* */
     uint32_t max_leaf_size;          /* BVH leaf size cap (0 = 4; at most 8) */
     uint32_t max_depth;              /* BVH depth cap (0 = 32; at most 64) */
+    float    traversal_cost;         /* SAH cost of a node visit relative to one primitive test (0 = 3) */
 } vkr_scene_desc;
 
 /* Replaces the SceneData SSBO creation and upload
@@ -310,6 +311,10 @@ typedef struct vkr_counters {
     uint64_t rays_traced;     /* rays actually traversed: shadow rays whose contribution is
                                  exactly 0 whatever their outcome are counted in `rays`
                                  but not traversed */
+    uint64_t wave_node_iters; /* wave-level BVH node-loop iterations (VKR_CTX_COUNT_WORK);
+                                 SIMD efficiency = nodes_visited / (64 * wave_node_iters) */
+    uint64_t wave_leaf_iters; /* wave-level leaf visits (VKR_CTX_COUNT_WORK) */
+    uint64_t wave_steps;      /* wave-level state-machine steps (VKR_CTX_COUNT_WORK) */
 } vkr_counters;
 
 vkr_error vkr_last_counters(vkr_ctx* ctx, vkr_counters* out);

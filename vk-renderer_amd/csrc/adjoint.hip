@@ -70,7 +70,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(const AdjointLaunch A)
     uint32_t ei = 0;
     bool new_path = false;
     uint32_t c_rays = 0;
-    WorkCnt wc{0, 0, 0, 0, 0};
+    WorkCnt wc{};
 
     auto claim = [&](uint32_t g) __attribute__((always_inline)) {
         const uint32_t tile = g >> 6, within = g & 63u;

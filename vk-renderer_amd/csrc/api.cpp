@@ -107,6 +107,7 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
     bi.n_spheres = desc->n_spheres;
     bi.max_leaf = desc->max_leaf_size ? desc->max_leaf_size : 4u;
     bi.max_depth = desc->max_depth ? desc->max_depth : 32u;
+    bi.traversal_cost = desc->traversal_cost;
     BuildOutput bo = build_bvh(bi);
     const auto t1 = std::chrono::steady_clock::now();
 
@@ -310,6 +311,9 @@ vkr_error vkr_last_counters(vkr_ctx* ctx, vkr_counters* out)
     out->planes_tested = c.planes;
     out->window_overflow = c.overflow;
     out->rays_traced = c.traced;
+    out->wave_node_iters = c.wave_nodes;
+    out->wave_leaf_iters = c.wave_leaves;
+    out->wave_steps = c.wave_steps;
     return VKR_SUCCESS;
 }
 

@@ -52,6 +52,7 @@ struct BuildInput {
     uint32_t n_spheres = 0;
     uint32_t max_leaf = 4;
     uint32_t max_depth = 32;
+    float traversal_cost = 1.0f;  // SAH cost of a node visit relative to one primitive test
 };
 
 struct BuildOutput {

@@ -52,6 +52,7 @@ struct JobDesc {
 
 struct DevCounters {
     unsigned long long rays, shadow, traced, nodes, tris, spheres, planes, overflow;
+    unsigned long long wave_nodes, wave_leaves, wave_steps;
 };
 
 struct PathLaunch {

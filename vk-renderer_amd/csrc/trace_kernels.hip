@@ -48,9 +48,9 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
     uint32_t ei = 0;
     uint32_t rays_px = 0, prim0 = kMiss, drec = 0, drep = 0;
     float t0 = 0.0f;
-    uint32_t c_rays = 0, c_shadow = 0, c_traced = 0;
+    uint32_t c_rays = 0, c_shadow = 0, c_traced = 0, wave_steps = 0;
     bool new_path = false;
-    WorkCnt wc{0, 0, 0, 0, 0};
+    WorkCnt wc{};
 
     // Claims pool index g for this lane (no-op for indices outside the image).
     auto claim = [&](uint32_t g) __attribute__((always_inline)) {
@@ -113,6 +113,7 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
             if (exhausted) break;
             continue;
         }
+        if (COUNT && __lane_id() == 0) ++wave_steps;
 
         const JobDesc& j = BATCH ? A.jobs[job] : A.job;
         if (has && new_path) {
@@ -145,50 +146,62 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
                 o = ro;
                 d = rd;
                 need = true;
+                ++rays_px;
+                ++c_rays;
+                ++c_traced;
             } else {
-                // Next-event sample of emitter ei (:464-469) and its
-                // unoccluded contribution (:475-502), evaluated before the
-                // shadow ray: when it is exactly +-0 the ray's outcome cannot
-                // change e, so it is not traversed (it is still counted).
-                const vkr_sphere* S = j.sc.spheres + j.sc.emitters[ei];
-                const f3 sP = ld3(S->P);
-                const float sr = S->r;
-                const float tl = length(sub(sP, P)) - sr;
-                const f3 l0 = sub(sP, P);
-                const float cos_a_max = sqrtf(1.0f - gclamp(sr * sr / dot(l0, l0), 0.0f, 1.0f));
-                const float cosa = gmix(cos_a_max, 1.0f, rng_next(salt, uvx, uvy));
-                const float phi = kTwoPi * rng_next(salt, uvx, uvy);
-                const f3 L = jitter(l0, phi, sqrtf(1.0f - cosa * cosa), cosa);
-
-                const float x = tl / sr + 1.0f;
-                f3 att = divs(muls(ld3(S->mat.emissive), 1.0f), x * x);
-                att = divs(subs(att, kAttSub), kAttDiv);
-                att = maxs(att, 0.0f);
+                // Next-event samples (:464-469) and their unoccluded
+                // contributions (:475-502), evaluated before the shadow ray.
+                // A contribution of exactly +-0 cannot change e whatever the
+                // ray's outcome, so that ray is counted but not traversed and
+                // the lane moves straight on to the next emitter (same RNG
+                // order); the step ends at the first ray that must be traced,
+                // or with the emitter list done.
                 const f3 alb = ld3(hm->albedo);
                 const float rough = hm->roughness, metal = hm->metalness;
-                const f3 F0 = mix3(f3{0.04f, 0.04f, 0.04f}, alb, metal);
                 const f3 V = normalize(sub(ld3(j.cam_pos), P));
-                const f3 H = normalize(add(V, L));
-                const float NDF = distribution_ggx(N, H, rough);
-                const float G = geometry_smith(N, V, L, rough);
-                const f3 F = fresnel_schlick(gmax(dot(H, V), 0.0f), F0);
-                f3 kD = ssub(1.0f, F);
-                kD = muls(kD, 1.0f - metal);
-                const f3 numerator = muls(F, NDF * G);
-                const float denominator = 4.0f * gmax(dot(N, V), 0.0f) * gmax(dot(N, L), 0.0f);
-                const f3 specular = divs(numerator, gmax(denominator, 0.001f));
-                const float NdotL = gmax(dot(N, L), 0.0f);
-                cnee = muls(mul(add(divs(mul(kD, alb), kPi), specular), att), NdotL);
-                need = !(cnee.x == 0.0f && cnee.y == 0.0f && cnee.z == 0.0f);
-                o = P;
-                d = L;
-                tmax = tl;
-                any = true;
-                ++c_shadow;
+                while (true) {
+                    const vkr_sphere* S = j.sc.spheres + j.sc.emitters[ei];
+                    const f3 sP = ld3(S->P);
+                    const float sr = S->r;
+                    const float tl = length(sub(sP, P)) - sr;
+                    const f3 l0 = sub(sP, P);
+                    const float cos_a_max = sqrtf(1.0f - gclamp(sr * sr / dot(l0, l0), 0.0f, 1.0f));
+                    const float cosa = gmix(cos_a_max, 1.0f, rng_next(salt, uvx, uvy));
+                    const float phi = kTwoPi * rng_next(salt, uvx, uvy);
+                    const f3 L = jitter(l0, phi, sqrtf(1.0f - cosa * cosa), cosa);
+
+                    const float x = tl / sr + 1.0f;
+                    f3 att = divs(muls(ld3(S->mat.emissive), 1.0f), x * x);
+                    att = divs(subs(att, kAttSub), kAttDiv);
+                    att = maxs(att, 0.0f);
+                    const f3 F0 = mix3(f3{0.04f, 0.04f, 0.04f}, alb, metal);
+                    const f3 H = normalize(add(V, L));
+                    const float NDF = distribution_ggx(N, H, rough);
+                    const float G = geometry_smith(N, V, L, rough);
+                    const f3 F = fresnel_schlick(gmax(dot(H, V), 0.0f), F0);
+                    f3 kD = ssub(1.0f, F);
+                    kD = muls(kD, 1.0f - metal);
+                    const f3 numerator = muls(F, NDF * G);
+                    const float denominator = 4.0f * gmax(dot(N, V), 0.0f) * gmax(dot(N, L), 0.0f);
+                    const f3 specular = divs(numerator, gmax(denominator, 0.001f));
+                    const float NdotL = gmax(dot(N, L), 0.0f);
+                    cnee = muls(mul(add(divs(mul(kD, alb), kPi), specular), att), NdotL);
+                    ++rays_px;
+                    ++c_rays;
+                    ++c_shadow;
+                    need = !(cnee.x == 0.0f && cnee.y == 0.0f && cnee.z == 0.0f);
+                    if (need) {
+                        ++c_traced;
+                        o = P;
+                        d = L;
+                        tmax = tl;
+                        any = true;
+                        break;
+                    }
+                    if (++ei >= j.sc.n_emitters) break;
+                }
             }
-            ++rays_px;
-            ++c_rays;
-            if (need) ++c_traced;
         }
 
         TravOut tr{kMiss, 0.0f, false};
@@ -254,8 +267,10 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
                 }
             }
         } else {
-            if (need && !tr.hit) e = add(e, cnee);                   // :473, :502
-            ++ei;
+            if (need) {
+                if (!tr.hit) e = add(e, cnee);                       // :473, :502
+                ++ei;
+            }
             if (ei >= j.sc.n_emitters) fin_diffuse = true;
         }
 
@@ -323,6 +338,9 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
         flush_counter(&A.counters->tris, wc.tris);
         flush_counter(&A.counters->spheres, wc.spheres);
         flush_counter(&A.counters->planes, wc.planes);
+        flush_counter(&A.counters->wave_nodes, wc.wave_nodes);
+        flush_counter(&A.counters->wave_leaves, wc.wave_leaves);
+        flush_counter(&A.counters->wave_steps, wave_steps);
     }
 }
 
@@ -579,6 +597,7 @@ __global__ void k_zero(DevCounters* c, uint32_t* pool)
 {
     if (threadIdx.x == 0) {
         c->rays = c->shadow = c->traced = c->nodes = c->tris = c->spheres = c->planes = c->overflow = 0ull;
+        c->wave_nodes = c->wave_leaves = c->wave_steps = 0ull;
         if (pool) *pool = 0u;
     }
 }

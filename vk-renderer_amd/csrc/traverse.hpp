@@ -24,7 +24,15 @@ constexpr int kCand = 4;
 
 struct WorkCnt {
     uint32_t nodes, tris, spheres, planes, overflow;
+    uint32_t wave_nodes, wave_leaves;   // wave-level loop iterations (counting variant)
 };
+
+// Counts one wave-level iteration (on the first active lane).
+__device__ __forceinline__ void count_wave(uint32_t& c)
+{
+    const unsigned long long act = __ballot(1);
+    if (__lane_id() == __ffsll((long long)act) - 1) ++c;
+}
 
 struct TravOut {
     uint32_t prim;  // (kind << 30) | index, kMiss when nothing was accepted
@@ -75,7 +83,10 @@ __device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, floa
                 const float4* np = reinterpret_cast<const float4*>(nodes + ref);
                 const float4 A = np[0], B = np[1], C = np[2];
                 const uint4 D = reinterpret_cast<const uint4*>(np)[3];
-                if (COUNT) wc.nodes++;
+                if (COUNT) {
+                    wc.nodes++;
+                    count_wave(wc.wave_nodes);
+                }
                 const float cb = cull + fabsf(cull) * 1e-5f + 1e-5f;
                 const float a0 = __builtin_fmaf(A.x, ix, -ox), a1 = __builtin_fmaf(A.y, ix, -ox);
                 const float a2 = __builtin_fmaf(A.z, iy, -oy), a3 = __builtin_fmaf(A.w, iy, -oy);
@@ -107,6 +118,7 @@ __device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, floa
             if (ref == kDone) break;
 
             // Leaf: test its primitives.
+            if (COUNT) count_wave(wc.wave_leaves);
             const uint32_t count = ((ref >> kRefCountShift) & 7u) + 1u;
             const uint32_t first = ref & kRefFirstMask;
             for (uint32_t i = 0; i < count; ++i) {

@@ -74,7 +74,8 @@ class SceneDesc(C.Structure):
                 ("planes", C.c_void_p), ("n_planes", C.c_uint32),
                 ("tri_material", C.c_void_p),
                 ("materials", C.c_void_p), ("n_materials", C.c_uint32),
-                ("flags", C.c_uint32), ("max_leaf_size", C.c_uint32), ("max_depth", C.c_uint32)]
+                ("flags", C.c_uint32), ("max_leaf_size", C.c_uint32), ("max_depth", C.c_uint32),
+                ("traversal_cost", C.c_float)]
 
 
 class SceneInfo(C.Structure):
@@ -98,7 +99,8 @@ class Outputs(C.Structure):
 class Counters(C.Structure):
     _fields_ = [("rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("nodes_visited", C.c_uint64),
                 ("tris_tested", C.c_uint64), ("spheres_tested", C.c_uint64), ("planes_tested", C.c_uint64),
-                ("window_overflow", C.c_uint64), ("rays_traced", C.c_uint64)]
+                ("window_overflow", C.c_uint64), ("rays_traced", C.c_uint64), ("wave_node_iters", C.c_uint64),
+                ("wave_leaf_iters", C.c_uint64), ("wave_steps", C.c_uint64)]
 
 
 class FrameData(C.Structure):
@@ -248,14 +250,14 @@ class Context:
     def set_flags(self, flags: int):
         check(self.lib.vkr_set_ctx_flags(self.h, flags), "vkr_set_ctx_flags")
 
-    def upload(self, scene: Scene, max_leaf: int = 0, max_depth: int = 0) -> int:
+    def upload(self, scene: Scene, max_leaf: int = 0, max_depth: int = 0, traversal_cost: float = 0.0) -> int:
         tris = np.ascontiguousarray(scene.tris, TRIANGLE)
         sph = np.ascontiguousarray(scene.spheres, SPHERE)
         pln = np.ascontiguousarray(scene.planes, PLANE)
         tm = np.ascontiguousarray(np.asarray(scene.tri_material, MATERIAL).reshape(1))
         mats = np.ascontiguousarray(scene.materials, MATERIAL)
         d = SceneDesc(_ptr(tris), len(tris), _ptr(sph), len(sph), _ptr(pln), len(pln), _ptr(tm),
-                      _ptr(mats), len(mats), scene.flags, max_leaf, max_depth)
+                      _ptr(mats), len(mats), scene.flags, max_leaf, max_depth, traversal_cost)
         sid = C.c_uint32()
         check(self.lib.vkr_scene_upload(self.h, C.byref(d), C.byref(sid)), "vkr_scene_upload")
         return sid.value
