@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--cpu-pixels", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--workload", default="soup100k", choices=["soup100k", "batch64"])
     ap.add_argument("--scenes", type=int, default=64, help="batch64: number of scenes")
     return ap.parse_args()
@@ -45,6 +46,43 @@ def parse():
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+METRIC = "Mrays/s (all bounces) at 1080p 100k-tri, plain vs batched vs adjoint, 1 GPU"
+
+
+def bench_adjoint(ctx, torch, repeats=5):
+    """Adjoint workload (256x256, 16 spp, 4 bounces, 32 diffuse/metal spheres
+    + 1 light): forward radiance pass vs the path-replay adjoint pass with a
+    dense dLoss/dradiance, each timed over `repeats` launches."""
+    import scenes
+
+    wl = scenes.adjoint_scene()
+    sid = ctx.upload(wl.scene)
+    n = wl.width * wl.height
+    rad = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+    g4 = torch.ones((n, 4), dtype=torch.float32, device="cuda")
+    d = torch.zeros(7 * len(wl.scene.spheres) + 1, dtype=torch.float32, device="cuda")
+    p = wl.params()
+    res = {}
+    for name, fn in (("forward", lambda: ctx.trace(sid, wl.frame, p, radiance=rad)),
+                     ("adjoint", lambda: ctx.adjoint(sid, wl.frame, p, g4, d))):
+        fn()
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        for _ in range(repeats):
+            fn()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - ts) / repeats
+    ctx.trace(sid, wl.frame, p, radiance=rad)
+    torch.cuda.synchronize()
+    rays = ctx.counters()["rays"]
+    ctx.destroy_scene(sid)
+    return {"workload": "adjoint_256x256_16spp_4bounce_33spheres", "forward_ms": round(res["forward"] * 1e3, 3),
+            "adjoint_ms": round(res["adjoint"] * 1e3, 3),
+            "adjoint_over_forward": round(res["adjoint"] / res["forward"], 3),
+            "forward_mrays": round(rays / res["forward"] / 1e6, 2),
+            "adjoint_mrays": round(rays / res["adjoint"] / 1e6, 2)}
 
 
 def bench_batch(a):
@@ -94,7 +132,7 @@ def bench_batch(a):
     mr = {k: round(rays / v / 1e6, 2) for k, v in res.items()}
     log(f"seconds per batch {res}\nMrays/s {mr}")
     out = {
-        "metric": "Mrays/s (all bounces), 64-scene batched persistent launch, 1 GPU",
+        "metric": METRIC,
         "value": mr["batched_graph"], "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps, "warmup": 1,
         "ms_per_step": round(res["batched_graph"] * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded PCG64 scenes)",
@@ -202,6 +240,11 @@ def main():
             variants[name] = round(ctx.counters()["rays"] * 3 / (time.perf_counter() - ts) / 1e6, 2)
         log(f"variants Mrays/s: {variants}")
 
+    adjoint = None
+    if not a.no_adjoint:
+        adjoint = bench_adjoint(ctx, torch)
+        log(f"adjoint: {adjoint}")
+
     cpu = None
     if not a.no_cpu_baseline:
         import oracle_ffi
@@ -218,7 +261,7 @@ def main():
         log(f"cpu baseline: {cpu}")
 
     out = {
-        "metric": "Mrays/s (all bounces) at 1080p 100k-tri, 1 GPU",
+        "metric": METRIC,
         "value": round(value, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -239,6 +282,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "variants_mrays": variants,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

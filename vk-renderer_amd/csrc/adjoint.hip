@@ -59,7 +59,6 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(const AdjointLaunch A)
     }
 
     bool has = false, exhausted = false;
-    uint32_t px = 0, py = 0;
     float uvx = 0.0f, uvy = 0.0f, salt = 0.0f, salt0 = 0.0f;
     uint32_t smp = 0, depth = 0, phase = kPhTrace, pass = 1, zb = 0, nchain = 0, cstar = 0, hs = kMiss;
     f3 ro{}, rd{}, acc{}, mask{}, P{}, N{}, dj{}, e{}, cnee{}, gp{};
@@ -78,8 +77,6 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(const AdjointLaunch A)
         const uint32_t x = (tile % j.tiles_x) * 8u + (within & 7u);
         const uint32_t y = (tile / j.tiles_x) * 8u + (within >> 3);
         if (x >= j.W || y >= j.H) return;
-        px = x;
-        py = y;
         has = true;
         uvx = (float)x / (float)j.W;
         uvy = (float)y / (float)j.H;

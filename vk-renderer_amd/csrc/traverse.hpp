@@ -18,6 +18,16 @@
 namespace vkr {
 
 constexpr uint32_t kDone = 0xFFFFFFFFu;
+
+
+// A wave leaves the node loop for the leaf pass once at least kLeafExit8/8
+// of its lanes that are still traversing wait at a leaf; the others keep
+// their place in the tree and go on in the next round.  Every lane still
+// visits the same nodes and primitives in the same order (results do not
+// depend on it); only the grouping of lanes into wave iterations changes.
+// 5/8 measured best on incoherent 1080p / 100k-triangle frames (8/8 is the
+// plain while-while loop; 14% -> 27% node-loop SIMD efficiency).
+constexpr uint32_t kLeafExit8 = 5;
 constexpr uint32_t kPhTrace = 0;
 constexpr uint32_t kPhNee = 1;
 constexpr int kCand = 4;
@@ -78,100 +88,112 @@ __device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, floa
         const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
         uint32_t ref = 0;
         int sp = 0;
-        while (ref != kDone) {
-            while (!(ref & kRefLeaf)) {
-                const float4* np = reinterpret_cast<const float4*>(nodes + ref);
-                const float4 A = np[0], B = np[1], C = np[2];
-                const uint4 D = reinterpret_cast<const uint4*>(np)[3];
-                if (COUNT) {
-                    wc.nodes++;
-                    count_wave(wc.wave_nodes);
+        bool live = true;
+        while (true) {
+            while (true) {
+                const bool in_node = live && !(ref & kRefLeaf);
+                const unsigned long long mn = __ballot(in_node);
+                if (mn == 0) break;
+                {
+                    const unsigned long long ml = __ballot(live && ref != kDone && (ref & kRefLeaf));
+                    const uint32_t nl = (uint32_t)__popcll(ml), nn = (uint32_t)__popcll(mn);
+                    if (nl > 0 && nl * 8u >= (nl + nn) * kLeafExit8) break;
                 }
-                const float cb = cull + fabsf(cull) * 1e-5f + 1e-5f;
-                const float a0 = __builtin_fmaf(A.x, ix, -ox), a1 = __builtin_fmaf(A.y, ix, -ox);
-                const float a2 = __builtin_fmaf(A.z, iy, -oy), a3 = __builtin_fmaf(A.w, iy, -oy);
-                const float a4 = __builtin_fmaf(B.x, iz, -oz), a5 = __builtin_fmaf(B.y, iz, -oz);
-                const float b0 = __builtin_fmaf(B.z, ix, -ox), b1 = __builtin_fmaf(B.w, ix, -ox);
-                const float b2 = __builtin_fmaf(C.x, iy, -oy), b3 = __builtin_fmaf(C.y, iy, -oy);
-                const float b4 = __builtin_fmaf(C.z, iz, -oz), b5 = __builtin_fmaf(C.w, iz, -oz);
-                const float n0 = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), 0.0f));
-                const float f0 = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), cb));
-                const float n1 = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), 0.0f));
-                const float f1 = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), cb));
-                const bool h0 = n0 <= f0, h1 = n1 <= f1;
-                if (h0 && h1) {
+                if (in_node) {
+                    const float4* np = reinterpret_cast<const float4*>(nodes + ref);
+                    const float4 A = np[0], B = np[1], C = np[2];
+                    const uint4 D = reinterpret_cast<const uint4*>(np)[3];
+                    if (COUNT) {
+                        wc.nodes++;
+                        count_wave(wc.wave_nodes);
+                    }
+                    const float cb = cull + fabsf(cull) * 1e-5f + 1e-5f;
+                    const float a0 = __builtin_fmaf(A.x, ix, -ox), a1 = __builtin_fmaf(A.y, ix, -ox);
+                    const float a2 = __builtin_fmaf(A.z, iy, -oy), a3 = __builtin_fmaf(A.w, iy, -oy);
+                    const float a4 = __builtin_fmaf(B.x, iz, -oz), a5 = __builtin_fmaf(B.y, iz, -oz);
+                    const float b0 = __builtin_fmaf(B.z, ix, -ox), b1 = __builtin_fmaf(B.w, ix, -ox);
+                    const float b2 = __builtin_fmaf(C.x, iy, -oy), b3 = __builtin_fmaf(C.y, iy, -oy);
+                    const float b4 = __builtin_fmaf(C.z, iz, -oz), b5 = __builtin_fmaf(C.w, iz, -oz);
+                    const float n0 = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), 0.0f));
+                    const float f0 = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), cb));
+                    const float n1 = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), 0.0f));
+                    const float f1 = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), cb));
+                    const bool h0 = n0 <= f0, h1 = n1 <= f1;
+                    // Branch-free step: the far child is always written to
+                    // the free slot above the top (sp < STACK at any inner
+                    // node, since the tree depth bounds the stack), and only
+                    // counted as pushed when both children are hit.
                     const bool first0 = n0 <= n1;
-                    const uint32_t nearr = first0 ? D.x : D.y;
-                    const uint32_t farr = first0 ? D.y : D.x;
-                    stk[sp * BLOCK] = farr;
-                    ++sp;
-                    ref = nearr;
-                } else if (h0) {
-                    ref = D.x;
-                } else if (h1) {
-                    ref = D.y;
+                    const uint32_t nearr = (h0 && (!h1 || first0)) ? D.x : D.y;
+                    stk[sp * BLOCK] = first0 ? D.y : D.x;
+                    const uint32_t top = stk[(sp > 0 ? sp - 1 : 0) * BLOCK];
+                    const bool none = !h0 && !h1;
+                    ref = none ? (sp > 0 ? top : kDone) : nearr;
+                    sp = (h0 && h1) ? sp + 1 : (none ? (sp > 0 ? sp - 1 : 0) : sp);
+                }
+            }
+            if (ref == kDone) live = false;
+            if (__ballot(live) == 0) break;
+            if (live && (ref & kRefLeaf)) {
+                // Leaf: test its primitives.
+                if (COUNT) count_wave(wc.wave_leaves);
+                const uint32_t count = ((ref >> kRefCountShift) & 7u) + 1u;
+                const uint32_t first = ref & kRefFirstMask;
+                for (uint32_t i = 0; i < count; ++i) {
+                    float t;
+                    uint32_t key;
+                    if (ref & kRefSphere) {
+                        const float4 s4 = sph[first + i];
+                        if (COUNT) wc.spheres++;
+                        t = sphere_t(o, d, f3{s4.x, s4.y, s4.z}, s4.w);
+                        key = 0x80000000u | sph_key[first + i];
+                    } else {
+                        const float4* tp = reinterpret_cast<const float4*>(tris + first + i);
+                        const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+                        if (COUNT) wc.tris++;
+                        t = tri_t(o, d, f3{p0.x, p0.y, p0.z}, f3{p1.x, p1.y, p1.z}, f3{p2.x, p2.y, p2.z}, eps);
+                        key = __float_as_uint(p0.w);
+                    }
+                    if (!(t > eps)) continue;
+                    if (any) {
+                        if (t < tmax_eps) occluded = true;
+                    } else if (t < fminf(tmin, tmax_eps) + win) {
+                        if (t < tmin) {
+                            tmin = t;
+                            const float lim = tmin + win;
+                            ct0 = ct0 < lim ? ct0 : inf;
+                            ct1 = ct1 < lim ? ct1 : inf;
+                            ct2 = ct2 < lim ? ct2 : inf;
+                            ct3 = ct3 < lim ? ct3 : inf;
+                        }
+                        if (ct0 == inf) { ct0 = t; ck0 = key; }
+                        else if (ct1 == inf) { ct1 = t; ck1 = key; }
+                        else if (ct2 == inf) { ct2 = t; ck2 = key; }
+                        else if (ct3 == inf) { ct3 = t; ck3 = key; }
+                        else {
+                            // List full: keep the four smallest t (documented near-tie).
+                            ovf = true;
+                            float mx = ct0; int mi = 0;
+                            if (ct1 > mx) { mx = ct1; mi = 1; }
+                            if (ct2 > mx) { mx = ct2; mi = 2; }
+                            if (ct3 > mx) { mx = ct3; mi = 3; }
+                            if (t < mx) {
+                                if (mi == 0) { ct0 = t; ck0 = key; }
+                                else if (mi == 1) { ct1 = t; ck1 = key; }
+                                else if (mi == 2) { ct2 = t; ck2 = key; }
+                                else { ct3 = t; ck3 = key; }
+                            }
+                        }
+                        cull = fminf(tmin, tmax_eps) + win;
+                    }
+                }
+                if (occluded) {
+                    live = false;
                 } else {
                     ref = sp > 0 ? stk[(sp - 1) * BLOCK] : kDone;
                     sp = sp > 0 ? sp - 1 : 0;
                 }
             }
-            if (ref == kDone) break;
-
-            // Leaf: test its primitives.
-            if (COUNT) count_wave(wc.wave_leaves);
-            const uint32_t count = ((ref >> kRefCountShift) & 7u) + 1u;
-            const uint32_t first = ref & kRefFirstMask;
-            for (uint32_t i = 0; i < count; ++i) {
-                float t;
-                uint32_t key;
-                if (ref & kRefSphere) {
-                    const float4 s4 = sph[first + i];
-                    if (COUNT) wc.spheres++;
-                    t = sphere_t(o, d, f3{s4.x, s4.y, s4.z}, s4.w);
-                    key = 0x80000000u | sph_key[first + i];
-                } else {
-                    const float4* tp = reinterpret_cast<const float4*>(tris + first + i);
-                    const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
-                    if (COUNT) wc.tris++;
-                    t = tri_t(o, d, f3{p0.x, p0.y, p0.z}, f3{p1.x, p1.y, p1.z}, f3{p2.x, p2.y, p2.z}, eps);
-                    key = __float_as_uint(p0.w);
-                }
-                if (!(t > eps)) continue;
-                if (any) {
-                    if (t < tmax_eps) occluded = true;
-                } else if (t < fminf(tmin, tmax_eps) + win) {
-                    if (t < tmin) {
-                        tmin = t;
-                        const float lim = tmin + win;
-                        ct0 = ct0 < lim ? ct0 : inf;
-                        ct1 = ct1 < lim ? ct1 : inf;
-                        ct2 = ct2 < lim ? ct2 : inf;
-                        ct3 = ct3 < lim ? ct3 : inf;
-                    }
-                    if (ct0 == inf) { ct0 = t; ck0 = key; }
-                    else if (ct1 == inf) { ct1 = t; ck1 = key; }
-                    else if (ct2 == inf) { ct2 = t; ck2 = key; }
-                    else if (ct3 == inf) { ct3 = t; ck3 = key; }
-                    else {
-                        // List full: keep the four smallest t (documented near-tie).
-                        ovf = true;
-                        float mx = ct0; int mi = 0;
-                        if (ct1 > mx) { mx = ct1; mi = 1; }
-                        if (ct2 > mx) { mx = ct2; mi = 2; }
-                        if (ct3 > mx) { mx = ct3; mi = 3; }
-                        if (t < mx) {
-                            if (mi == 0) { ct0 = t; ck0 = key; }
-                            else if (mi == 1) { ct1 = t; ck1 = key; }
-                            else if (mi == 2) { ct2 = t; ck2 = key; }
-                            else { ct3 = t; ck3 = key; }
-                        }
-                    }
-                    cull = fminf(tmin, tmax_eps) + win;
-                }
-            }
-            if (occluded) break;
-            ref = sp > 0 ? stk[(sp - 1) * BLOCK] : kDone;
-            sp = sp > 0 ? sp - 1 : 0;
         }
     }
 
