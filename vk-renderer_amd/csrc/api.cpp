@@ -91,6 +91,8 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
         return fail(VKR_INVALID_ARGUMENT, "primitive array is NULL with a non-zero count");
     if (desc->n_tris > kRefFirstMask || desc->n_spheres > kRefFirstMask)
         return fail(VKR_INVALID_ARGUMENT, "too many primitives (limit 2^27 per kind)");
+    if ((uint64_t)desc->n_tris * sizeof(TriRecord) >= (1ull << 32))
+        return fail(VKR_INVALID_ARGUMENT, "too many triangles (triangle records are addressed with 32-bit offsets)");
     if ((desc->flags & VKR_SCENE_TRI_MATERIAL_INDEX) && (!desc->materials || desc->n_materials == 0))
         return fail(VKR_INVALID_ARGUMENT, "per-triangle materials requested without a material table");
     if (desc->max_leaf_size > kMaxLeaf) return fail(VKR_INVALID_ARGUMENT, "max_leaf_size must be <= 8");
@@ -105,7 +107,7 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
     bi.n_tris = desc->n_tris;
     bi.sph = reinterpret_cast<const float*>(desc->spheres);
     bi.n_spheres = desc->n_spheres;
-    bi.max_leaf = desc->max_leaf_size ? desc->max_leaf_size : 4u;
+    bi.max_leaf = desc->max_leaf_size ? desc->max_leaf_size : kDefaultLeaf;
     bi.max_depth = desc->max_depth ? desc->max_depth : 32u;
     bi.traversal_cost = desc->traversal_cost;
     BuildOutput bo = build_bvh(bi);
@@ -113,6 +115,8 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
 
     const uint32_t stack = bo.empty ? 32u : stack_variant(bo.depth);
     if (stack == 0) return fail(VKR_UNSUPPORTED, "BVH deeper than 64 levels");
+    if (bo.nodes.size() >= (1u << 26))
+        return fail(VKR_UNSUPPORTED, "BVH has 2^26 nodes or more (nodes are addressed with 32-bit offsets)");
 
     sc->host_spheres.assign(desc->spheres, desc->spheres + desc->n_spheres);
     const std::vector<uint32_t> em = emitter_list(sc->host_spheres);

@@ -307,7 +307,7 @@ BuildOutput build_bvh(const BuildInput& in)
     if (prims.empty()) return out;
 
     Builder b;
-    b.max_leaf = std::max(1u, std::min(in.max_leaf ? in.max_leaf : 4u, kMaxLeaf));
+    b.max_leaf = std::max(1u, std::min(in.max_leaf ? in.max_leaf : kDefaultLeaf, kMaxLeaf));
     b.max_depth = std::max(8u, std::min(in.max_depth ? in.max_depth : 32u, 64u));
     // A node visit (four 128-bit loads, two slab tests, a stack step) costs
     // about three triangle tests on gfx950 (measured on the 100k soup).

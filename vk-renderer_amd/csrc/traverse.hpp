@@ -25,9 +25,10 @@ constexpr uint32_t kDone = 0xFFFFFFFFu;
 // their place in the tree and go on in the next round.  Every lane still
 // visits the same nodes and primitives in the same order (results do not
 // depend on it); only the grouping of lanes into wave iterations changes.
-// 5/8 measured best on incoherent 1080p / 100k-triangle frames (8/8 is the
-// plain while-while loop; 14% -> 27% node-loop SIMD efficiency).
-constexpr uint32_t kLeafExit8 = 5;
+// 4/8 measured best on incoherent 1080p / 100k-triangle frames with leaves
+// of at most 2 primitives (8/8 is the plain while-while loop: node-loop SIMD
+// efficiency 14% -> 31%).
+constexpr uint32_t kLeafExit8 = 4;
 constexpr uint32_t kPhTrace = 0;
 constexpr uint32_t kPhNee = 1;
 constexpr int kCand = 4;
@@ -100,7 +101,10 @@ __device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, floa
                     if (nl > 0 && nl * 8u >= (nl + nn) * kLeafExit8) break;
                 }
                 if (in_node) {
-                    const float4* np = reinterpret_cast<const float4*>(nodes + ref);
+                    // 32-bit byte offsets (node count < 2^26, checked at
+                    // upload) let the loads use a scalar base address.
+                    const float4* np =
+                        reinterpret_cast<const float4*>(reinterpret_cast<const char*>(nodes) + (ref << 6));
                     const float4 A = np[0], B = np[1], C = np[2];
                     const uint4 D = reinterpret_cast<const uint4*>(np)[3];
                     if (COUNT) {
@@ -148,7 +152,8 @@ __device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, floa
                         t = sphere_t(o, d, f3{s4.x, s4.y, s4.z}, s4.w);
                         key = 0x80000000u | sph_key[first + i];
                     } else {
-                        const float4* tp = reinterpret_cast<const float4*>(tris + first + i);
+                        const float4* tp = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(tris) +
+                                                                            (first + i) * (uint32_t)sizeof(TriRecord));
                         const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
                         if (COUNT) wc.tris++;
                         t = tri_t(o, d, f3{p0.x, p0.y, p0.z}, f3{p1.x, p1.y, p1.z}, f3{p2.x, p2.y, p2.z}, eps);
