@@ -160,6 +160,7 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
     d.n_spheres = desc->n_spheres;
     d.n_planes = desc->n_planes;
     d.n_emitters = (uint32_t)em.size();
+    d.emitters_bounded = emitters_bounded(sc->host_spheres, em);
     d.n_materials = nm;
     d.flags = desc->flags;
     d.has_bvh = bo.empty ? 0u : 1u;
@@ -236,6 +237,7 @@ vkr_error vkr_scene_set_sphere_params(vkr_ctx* ctx, uint32_t scene, const float*
         VKR_HIP(hipMemcpy(s->emitters.p, em.data(), em.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     }
     s->dev.n_emitters = (uint32_t)em.size();
+    s->dev.emitters_bounded = emitters_bounded(s->host_spheres, em);
     s->info.n_emitters = (uint32_t)em.size();
     s->host_theta.assign(params, params + n);
     VKR_HIP(hipMemcpy(s->theta.p, s->host_theta.data(), (size_t)n * sizeof(float), hipMemcpyHostToDevice));

@@ -25,6 +25,7 @@ struct DevScene {
     const uint32_t* emitters;       // scan indices of spheres with emissive != 0
     uint32_t n_tris, n_spheres, n_planes, n_emitters;
     uint32_t n_materials, flags, has_bvh, stack_needed;
+    uint32_t emitters_bounded;      // every emitter: |emissive| < 2^80, 2^-40 <= r <= 2^40
 };
 
 // One frame of work (a single trace call, or one job of a batch).

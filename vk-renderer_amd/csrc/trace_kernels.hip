@@ -50,6 +50,7 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
     float t0 = 0.0f;
     uint32_t c_rays = 0, c_shadow = 0, c_traced = 0, wave_steps = 0;
     bool new_path = false;
+    bool nee_zero = false;              // every emitter of this hit contributes exactly +-0
     WorkCnt wc{};
 
     // Claims pool index g for this lane (no-op for indices outside the image).
@@ -160,7 +161,25 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
                 const f3 alb = ld3(hm->albedo);
                 const float rough = hm->roughness, metal = hm->metalness;
                 const f3 V = normalize(sub(ld3(j.cam_pos), P));
-                while (true) {
+                if (nee_zero) {
+                    // Such emitters are counted and their two random draws
+                    // (:464-465) skipped by advancing the stream state alone.
+                    // A hit point within 2^-19 radii of an emitter's centre
+                    // (attenuation may overflow) is evaluated in full below.
+                    while (ei < j.sc.n_emitters) {
+                        const vkr_sphere* S = j.sc.spheres + j.sc.emitters[ei];
+                        const f3 l0 = sub(ld3(S->P), P);
+                        const float sr = S->r;
+                        if (!(dot(l0, l0) > (sr * sr) * 0x1p-38f)) break;
+                        salt = salt + 1.0f;
+                        salt = salt + 1.0f;
+                        ++rays_px;
+                        ++c_rays;
+                        ++c_shadow;
+                        ++ei;
+                    }
+                }
+                while (ei < j.sc.n_emitters) {
                     const vkr_sphere* S = j.sc.spheres + j.sc.emitters[ei];
                     const f3 sP = ld3(S->P);
                     const float sr = S->r;
@@ -242,6 +261,17 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
                     dj = muls(jitter(N, phi, sqrtf(r2), sqrtf(1.0f - r2)), 1.0f - m->metalness);
                     e = f3{0.0f, 0.0f, 0.0f};
                     ei = 0;
+                    // Metalness 1 and roughness 0 (the reference's `mirror`):
+                    // kD = (1 - F) * 0 and NDF = 0 / (pi * denom^2), where
+                    // denom = 1 - NdotH^2 >= 8e-5 once |N|^2 < 0.9999 (|H| = 1),
+                    // so with finite albedo, F, G and attenuation every
+                    // emitter's contribution is exactly +-0 whatever its sample.
+                    {
+                        const f3 alb = ld3(m->albedo);
+                        nee_zero = j.sc.emitters_bounded && m->metalness == 1.0f && m->roughness == 0.0f &&
+                                   fabsf(alb.x) < 1e30f && fabsf(alb.y) < 1e30f && fabsf(alb.z) < 1e30f &&
+                                   dot(N, N) < 0.9999f;
+                    }
                     if (j.sc.n_emitters > 0) phase = kPhNee;
                     else fin_diffuse = true;
                 } else if (type == VKR_MAT_DIELECTRIC) {             // :514-542
