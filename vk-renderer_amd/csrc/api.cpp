@@ -268,9 +268,10 @@ vkr_error vkr_trace_frame(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd
     DevCounters* cnt = ctx->counters.as<DevCounters>();
     uint32_t* pool = ctx->pool.as<uint32_t>();
     VKR_HIP(launch_zero_counters(cnt, pool, st));
-    VKR_HIP(hipEventRecord(ctx->ev0, st));
     if (params->kernel == VKR_KERNEL_WHITTED) {
+        VKR_HIP(hipEventRecord(ctx->ev0, st));
         VKR_HIP(launch_whitted(j, cnt, st));
+        VKR_HIP(hipEventRecord(ctx->ev1, st));
     } else {
         PathLaunch a{};
         a.job = j;
@@ -288,14 +289,15 @@ vkr_error vkr_trace_frame(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd
         if (!block) block = 128u;
         const bool persistent = sched != VKR_SCHED_PLAIN;
         a.persistent = persistent ? 1u : 0u;
-        a.chunk = chunk ? chunk : 32u;
+        a.chunk = chunk ? chunk : kDefaultChunk;
         const bool count = (ctx->flags & VKR_CTX_COUNT_WORK) != 0;
         const uint32_t grid = persistent ? std::min(persistent_grid(block, stack, count, false, ctx->device),
                                                     (a.total + block - 1u) / block)
                                          : (a.total + block - 1u) / block;
+        VKR_HIP(hipEventRecord(ctx->ev0, st));
         VKR_HIP(launch_path(a, block, stack, count, false, grid, st));
+        VKR_HIP(hipEventRecord(ctx->ev1, st));
     }
-    VKR_HIP(hipEventRecord(ctx->ev1, st));
     ctx->timed = true;
     ctx->last_counters = nullptr;
     return VKR_SUCCESS;

@@ -63,7 +63,7 @@ PathLaunch batch_launch(const BatchPlan& plan, const JobDesc* djobs, const uint3
     a.n_jobs = n;
     a.total = (uint32_t)plan.total;
     a.persistent = 1u;
-    a.chunk = chunk ? chunk : 32u;
+    a.chunk = chunk ? chunk : kDefaultChunk;
     a.pool = pool;
     a.counters = cnt;
     return a;
@@ -225,7 +225,7 @@ vkr_error vkr_autotune(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd, c
     for (uint32_t st : stacks) {
         if (st < s->dev.stack_needed) continue;
         for (uint32_t b : {64u, 128u, 256u})
-            for (uint32_t c : {1u, 8u, 32u, 64u}) cands.push_back({b, c, st, (uint32_t)VKR_SCHED_PERSISTENT});
+            for (uint32_t c : {1u, 8u, 16u, 32u}) cands.push_back({b, c, st, (uint32_t)VKR_SCHED_PERSISTENT});
     }
     // Drop any previous tuning of this (scene, size) so candidates run as given.
     ctx->tuned.erase(std::remove_if(ctx->tuned.begin(), ctx->tuned.end(),
@@ -242,7 +242,7 @@ vkr_error vkr_autotune(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd, c
         p.chunk = c.chunk;
         p.sched = c.sched;
         // The stack variant is chosen through a temporary tuning entry.
-        ctx->tuned.push_back(TuneEntry{scene, p.width, p.height, c.block, c.chunk ? c.chunk : 32u, c.stack, c.sched});
+        ctx->tuned.push_back(TuneEntry{scene, p.width, p.height, c.block, c.chunk ? c.chunk : kDefaultChunk, c.stack, c.sched});
         float total = 0.0f;
         vkr_error e = vkr_trace_frame(ctx, scene, fd, &p, outputs, nullptr);
         for (uint32_t r = 0; r < repeats && e == VKR_SUCCESS; ++r) {
@@ -267,7 +267,7 @@ vkr_error vkr_autotune(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd, c
             win = c;
         }
     }
-    ctx->tuned.push_back(TuneEntry{scene, params->width, params->height, win.block, win.chunk ? win.chunk : 32u,
+    ctx->tuned.push_back(TuneEntry{scene, params->width, params->height, win.block, win.chunk ? win.chunk : kDefaultChunk,
                                    win.stack, win.sched});
     out->block = win.block;
     out->chunk = win.chunk;

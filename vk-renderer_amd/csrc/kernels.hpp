@@ -56,6 +56,11 @@ struct DevCounters {
     unsigned long long wave_nodes, wave_leaves, wave_steps;
 };
 
+// Default pool refill threshold: a wave claims new pixels once at least this
+// many of its lanes are idle (8-16 measured best at 1080p; 32 idles lanes,
+// 1 refills too often).
+constexpr uint32_t kDefaultChunk = 8u;
+
 struct PathLaunch {
     JobDesc job;                  // single-frame launches
     const JobDesc* jobs;          // batched launches

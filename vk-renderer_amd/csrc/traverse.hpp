@@ -89,18 +89,13 @@ __device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, floa
         const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
         uint32_t ref = 0;
         int sp = 0;
-        bool live = true;
         while (true) {
             while (true) {
-                const bool in_node = live && !(ref & kRefLeaf);
-                const unsigned long long mn = __ballot(in_node);
-                if (mn == 0) break;
-                {
-                    const unsigned long long ml = __ballot(live && ref != kDone && (ref & kRefLeaf));
-                    const uint32_t nl = (uint32_t)__popcll(ml), nn = (uint32_t)__popcll(mn);
-                    if (nl > 0 && nl * 8u >= (nl + nn) * kLeafExit8) break;
-                }
-                if (in_node) {
+                // ref >= 0 (as int): inner node; < -1: leaf; -1 (kDone): finished.
+                const uint32_t nn = (uint32_t)__popcll(__ballot((int)ref >= 0));
+                const uint32_t nl = (uint32_t)__popcll(__ballot((int)ref < -1));
+                if (nn == 0 || (nl > 0 && nl * 8u >= (nl + nn) * kLeafExit8)) break;
+                if ((int)ref >= 0) {
                     // 32-bit byte offsets (node count < 2^26, checked at
                     // upload) let the loads use a scalar base address.
                     const float4* np =
@@ -136,9 +131,8 @@ __device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, floa
                     sp = (h0 && h1) ? sp + 1 : (none ? (sp > 0 ? sp - 1 : 0) : sp);
                 }
             }
-            if (ref == kDone) live = false;
-            if (__ballot(live) == 0) break;
-            if (live && (ref & kRefLeaf)) {
+            if (__ballot(ref != kDone) == 0) break;
+            if ((int)ref < -1) {
                 // Leaf: test its primitives.
                 if (COUNT) count_wave(wc.wave_leaves);
                 const uint32_t count = ((ref >> kRefCountShift) & 7u) + 1u;
@@ -193,7 +187,7 @@ __device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, floa
                     }
                 }
                 if (occluded) {
-                    live = false;
+                    ref = kDone;
                 } else {
                     ref = sp > 0 ? stk[(sp - 1) * BLOCK] : kDone;
                     sp = sp > 0 ? sp - 1 : 0;
