@@ -347,6 +347,36 @@ typedef struct {
     float prim_t;
 } path_ctx;
 
+/* Unoccluded next-event contribution of one emitter sample (Tracer.comp:475-502). */
+static v3 nee_contribution(v3 P, v3 N, v3 cam_pos, const omat* mat, const osph* sp, float t, v3 L)
+{
+    const v3 albedo = Vp(mat->albedo);
+    const float x = t / sp->r + 1.0f;
+    v3 att = vdivs(vmuls(Vp(sp->mat.emissive), 1.0f), x * x);   /* :475 */
+    att = vdivs(vsubs(att, K_ATT_SUB), K_ATT_DIV);              /* :478 */
+    att = vmaxs(att, 0.0f);                                     /* :479 */
+
+    v3 F0 = V(0.04f, 0.04f, 0.04f);
+    F0 = vmix(F0, albedo, mat->metalness);                      /* :482 */
+    const v3 Vv = vnormalize(vsub(cam_pos, P));                 /* :484 */
+    const v3 H = vnormalize(vadd(Vv, L));
+
+    const float NDF = distribution_ggx(N, H, mat->roughness);
+    const float G = geometry_smith(N, Vv, L, mat->roughness);
+    const v3 F = fresnel_schlick(gmax(vdot(H, Vv), 0.0f), F0);
+
+    const v3 kS = F;
+    v3 kD = svsub(1.0f, kS);
+    kD = vmuls(kD, 1.0f - mat->metalness);
+
+    const v3 numerator = vmuls(F, NDF * G);
+    const float denominator = 4.0f * gmax(vdot(N, Vv), 0.0f) * gmax(vdot(N, L), 0.0f);
+    const v3 specular = vdivs(numerator, gmax(denominator, 0.001f));
+
+    const float NdotL = gmax(vdot(N, L), 0.0f);
+    return vmuls(vmul(vadd(vdivs(vmul(kD, albedo), K_PI), specular), att), NdotL);
+}
+
 static v3 radiance(path_ctx* pc, rng_t* rng, v3 ro, v3 rd, uint16_t* decision)
 {
     const oscene* s = pc->s;
@@ -397,32 +427,8 @@ static v3 radiance(path_ctx* pc, rng_t* rng, v3 ro, v3 rd, uint16_t* decision)
                 memset(&sh, 0, sizeof(sh));
                 sh.t = t;
                 pc->rays++;
-                if (!trace_ray(s, it.P, L, &sh, pc->eps)) {
-                    const float x = t / sp->r + 1.0f;
-                    v3 att = vdivs(vmuls(Vp(sp->mat.emissive), 1.0f), x * x);   /* :475 */
-                    att = vdivs(vsubs(att, K_ATT_SUB), K_ATT_DIV);              /* :478 */
-                    att = vmaxs(att, 0.0f);                                     /* :479 */
-
-                    v3 F0 = V(0.04f, 0.04f, 0.04f);
-                    F0 = vmix(F0, albedo, mat->metalness);                      /* :482 */
-                    const v3 Vv = vnormalize(vsub(Vp(pc->f->camera.pos), it.P));  /* :484 */
-                    const v3 H = vnormalize(vadd(Vv, L));
-
-                    const float NDF = distribution_ggx(it.N, H, mat->roughness);
-                    const float G = geometry_smith(it.N, Vv, L, mat->roughness);
-                    const v3 F = fresnel_schlick(gmax(vdot(H, Vv), 0.0f), F0);
-
-                    const v3 kS = F;
-                    v3 kD = svsub(1.0f, kS);
-                    kD = vmuls(kD, 1.0f - mat->metalness);
-
-                    const v3 numerator = vmuls(F, NDF * G);
-                    const float denominator = 4.0f * gmax(vdot(it.N, Vv), 0.0f) * gmax(vdot(it.N, L), 0.0f);
-                    const v3 specular = vdivs(numerator, gmax(denominator, 0.001f));
-
-                    const float NdotL = gmax(vdot(it.N, L), 0.0f);
-                    e = vadd(e, vmuls(vmul(vadd(vdivs(vmul(kD, albedo), K_PI), specular), att), NdotL));
-                }
+                if (!trace_ray(s, it.P, L, &sh, pc->eps))
+                    e = vadd(e, nee_contribution(it.P, it.N, Vp(pc->f->camera.pos), mat, sp, t, L));
             }
 
             /* :507 when_gt(emissive, 0) == vec3(1) ? normalize(emissive) : 0 */
@@ -879,4 +885,15 @@ void vkro_camera_update(const float pos[3], float pitch, float yaw, ocam* out)
     out->dir[0] = dir.x; out->dir[1] = dir.y; out->dir[2] = dir.z;
     out->right[0] = right.x; out->right[1] = right.y; out->right[2] = right.z;
     out->up[0] = up.x; out->up[1] = up.y; out->up[2] = up.z;
+}
+
+void vkro_nee_contribution(const float P[3], const float N[3], const float cam_pos[3], const omat* mat,
+                           const osph* sp, const float L[3], float out[3])
+{
+    const v3 p = Vp(P);
+    const float t = vlength(vsub(Vp(sp->P), p)) - sp->r;                     /* :465 */
+    const v3 c = nee_contribution(p, Vp(N), Vp(cam_pos), mat, sp, t, Vp(L));
+    out[0] = c.x;
+    out[1] = c.y;
+    out[2] = c.z;
 }

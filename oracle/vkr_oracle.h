@@ -104,6 +104,10 @@ void  vkro_jitter(const float d[3], float phi, float sina, float cosa, float out
 void  vkro_reflect(const float I[3], const float N[3], float out[3]);
 void  vkro_refract(const float I[3], const float N[3], float eta, float out[3]);
 void  vkro_primary_dir(const oframe* frame, uint32_t x, uint32_t y, uint32_t w, uint32_t h, float out[3]);
+/* Unoccluded next-event contribution of one emitter sample L at hit (P, N)
+ * seen from cam_pos (Tracer.comp:465, 475-502). */
+void  vkro_nee_contribution(const float P[3], const float N[3], const float cam_pos[3], const omat* mat,
+                            const osph* sp, const float L[3], float out[3]);
 
 /* Present pass, Assets/Fullscreen.frag:14-31 with Fullscreen.vert:6-11. */
 int vkro_filter(const uint8_t* cur, const uint8_t* prev, uint32_t src_w, uint32_t src_h,

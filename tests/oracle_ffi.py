@@ -86,6 +86,8 @@ def load():
         lib.vkro_primary_dir.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, fp]
         lib.vkro_filter.restype = C.c_int
         lib.vkro_filter.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32]
+        lib.vkro_nee_contribution.restype = None
+        lib.vkro_nee_contribution.argtypes = [fp, fp, fp, C.c_void_p, C.c_void_p, fp, fp]
         lib.vkro_camera_update.restype = None
         lib.vkro_camera_update.argtypes = [fp, C.c_float, C.c_float, C.c_void_p]
         _lib = lib
