@@ -54,3 +54,29 @@ def test_draw_matches_oracle(tmp_path):
     exp = np.zeros((pw * ph, 4), np.uint8)
     oracle_ffi.load().vkro_filter(refs[0].ctypes.data, refs[1].ctypes.data, res, res, exp.ctypes.data, pw, ph)
     assert np.array_equal(pres, exp)
+
+
+@pytest.mark.gpu
+def test_pose_replay_matches_oracle(tmp_path):
+    """--poses replays recorded camera input: each frame's camera comes from
+    Camera::update of its pose (Source/Camera.cpp:91-107)."""
+    import torch
+    import vkr
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    res = 128
+    poses = [(32.8509, 30.6991, -106.389, 4.44998, -602.79), (10.0, 40.0, -90.0, -12.5, -575.0)]
+    pf = tmp_path / "poses.txt"
+    pf.write_text("# x y z pitch yaw\n" + "\n".join(" ".join(str(v) for v in p) for p in poses) + "\n")
+    prefix = str(tmp_path / "p")
+    r = _run(["--frames", "2", "--res", str(res), "--seeds", "0.25,0.5", "--poses", str(pf), "--raw", prefix])
+    assert r.returncode == 0, r.stdout + r.stderr
+    wl = scenes.reference_scene(res, res)
+    fr = wl.frame.copy()
+    fr["aspect_ratio"] = np.float32(1024) / np.float32(768)   # default present extent (Main.cpp:92)
+    for slot, (pose, seed) in enumerate(zip(poses, (0.25, 0.5))):
+        got = np.fromfile(prefix + f".traced{slot}.rgba", np.uint8).reshape(-1, 4)
+        fr["camera"] = vkr.camera_from_angles(pose[:3], pose[3], pose[4])
+        fr["seed"] = np.float32(seed)
+        ref = oracle_ffi.trace(wl.scene, fr, res, res, 4, 4)["rgba8"]
+        assert np.array_equal(got, ref), slot

@@ -221,6 +221,32 @@ def test_soup_pixel_subset(ctx, torch_cuda):
     assert st["prim_id"] == 1.0 and st["radiance_1e-4"] >= 0.99
 
 
+def test_far_sphere_imprecise_hits(ctx, torch_cuda):
+    """Rays from 2000 units away: the reference's sphere test accepts rays
+    that pass outside the sphere (rounding in its discriminant); the traversal
+    widens the sphere boxes per ray so it finds the same hits."""
+    wl = scenes.far_sphere()
+    sid = ctx.upload(wl.scene)
+    g = gpu_trace(ctx, sid, wl)
+    r = oracle_ffi.trace(wl.scene, wl.frame, wl.width, wl.height, wl.spp, wl.max_depth)
+    # Float64 distance from each primary ray to the sphere's centre.
+    W, H = wl.width, wl.height
+    y, x = np.divmod(np.arange(W * H), W)
+    cam = wl.frame["camera"]
+    d = (cam["dir"][None].astype(np.float64) + cam["right"][None] * (2.0 * x / W - 1.0)[:, None]
+         + cam["up"][None] * (2.0 * y / H - 1.0)[:, None])
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    oc = cam["pos"].astype(np.float64) - np.array(wl.scene.spheres[0]["P"], np.float64)
+    b = d @ oc
+    p = np.sqrt(np.maximum(oc @ oc - b * b, 0.0))
+    outside = (r["prim_id"] == (1 << 30)) & (p > 1.0001)
+    st = compare(g, r)
+    _report("far_sphere", st)
+    print(f"[far_sphere] oracle hits on rays passing outside the sphere: {int(outside.sum())}", flush=True)
+    assert outside.sum() > 100
+    assert st["prim_id"] == 1.0 and st["hit_t_bits"] == 1.0 and st["radiance_bits"] == 1.0 and st["ray_count"] == 1.0
+
+
 def test_heightfield_pixel_subset(ctx, torch_cuda):
     """2.09M-triangle fBm heightfield + 4096 metal/dielectric spheres, 8 spp,
     8 bounces at 1920x1080: the oracle's linear scan on a pixel sample."""

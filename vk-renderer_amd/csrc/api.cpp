@@ -165,6 +165,8 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
     d.flags = desc->flags;
     d.has_bvh = bo.empty ? 0u : 1u;
     d.stack_needed = stack;
+    for (int k = 0; k < 3; ++k) d.sph_c[k] = bo.sph_c[k];
+    d.sph_r = bo.sph_r;
 
     vkr_scene_info& in = sc->info;
     in.n_tris = desc->n_tris;

@@ -56,23 +56,56 @@ inline uint32_t make_leaf(uint32_t first, uint32_t count, bool sphere)
     return kRefLeaf | (sphere ? kRefSphere : 0u) | ((count - 1u) << kRefCountShift) | (first & kRefFirstMask);
 }
 
+// Centre and half-extent of [lo, hi] in float, with the extent rounded up so
+// that [c - e, c + e] contains [lo, hi] exactly.
+inline void centre_extent(float lo, float hi, float& c, float& e)
+{
+    const double cd = 0.5 * ((double)lo + (double)hi);
+    c = (float)cd;
+    const double ed = 0.5 * ((double)hi - (double)lo) + std::fabs(cd - (double)c);
+    e = (float)ed;
+    if ((double)e < ed) e = std::nextafter(e, std::numeric_limits<float>::infinity());
+}
+
 inline void set_child(BvhNode& n, int slot, const Box& b, uint32_t ref)
 {
+    float c[3], e[3];
+    for (int k = 0; k < 3; ++k) centre_extent(b.lo[k], b.hi[k], c[k], e[k]);
     if (slot == 0) {
-        n.a[0] = b.lo[0]; n.a[1] = b.hi[0]; n.a[2] = b.lo[1]; n.a[3] = b.hi[1];
-        n.b[0] = b.lo[2]; n.b[1] = b.hi[2];
+        n.a[0] = c[0]; n.a[1] = e[0]; n.a[2] = c[1]; n.a[3] = e[1];
+        n.b[0] = c[2]; n.b[1] = e[2];
     } else {
-        n.b[2] = b.lo[0]; n.b[3] = b.hi[0];
-        n.c[0] = b.lo[1]; n.c[1] = b.hi[1]; n.c[2] = b.lo[2]; n.c[3] = b.hi[2];
+        n.b[2] = c[0]; n.b[3] = e[0];
+        n.c[0] = c[1]; n.c[1] = e[1]; n.c[2] = c[2]; n.c[3] = e[2];
     }
     n.d[slot] = ref;
 }
 
-inline Box empty_box()
+// A child box every ray enters (centre 0, infinite extent).
+inline void set_child_always(BvhNode& n, int slot, uint32_t ref)
 {
-    Box b;
-    for (int k = 0; k < 3; ++k) b.lo[k] = b.hi[k] = kEmpty;
-    return b;
+    const float inf = std::numeric_limits<float>::infinity();
+    if (slot == 0) {
+        n.a[0] = 0.0f; n.a[1] = inf; n.a[2] = 0.0f; n.a[3] = inf;
+        n.b[0] = 0.0f; n.b[1] = inf;
+    } else {
+        n.b[2] = 0.0f; n.b[3] = inf;
+        n.c[0] = 0.0f; n.c[1] = inf; n.c[2] = 0.0f; n.c[3] = inf;
+    }
+    n.d[slot] = ref;
+}
+
+// A child box no ray enters (the point (1e30, 1e30, 1e30)).
+inline void set_child_never(BvhNode& n, int slot, uint32_t ref)
+{
+    if (slot == 0) {
+        n.a[0] = kEmpty; n.a[1] = 0.0f; n.a[2] = kEmpty; n.a[3] = 0.0f;
+        n.b[0] = kEmpty; n.b[1] = 0.0f;
+    } else {
+        n.b[2] = kEmpty; n.b[3] = 0.0f;
+        n.c[0] = kEmpty; n.c[1] = 0.0f; n.c[2] = kEmpty; n.c[3] = 0.0f;
+    }
+    n.d[slot] = ref;
 }
 
 struct Builder {
@@ -315,18 +348,61 @@ BuildOutput build_bvh(const BuildInput& in)
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
     b.threads_left = (int)std::min(hw, 16u) - 1;
 
+    // Triangles first, spheres after; each kind gets its own subtree.
+    const auto mid = std::stable_partition(prims.begin(), prims.end(), [](const Prim& q) { return q.sphere == 0; });
+    const size_t n_tri = (size_t)(mid - prims.begin()), n_sph = prims.size() - n_tri;
+    if (n_sph) {
+        double lo[3], hi[3];
+        for (int k = 0; k < 3; ++k) { lo[k] = 1e300; hi[k] = -1e300; }
+        for (size_t i = n_tri; i < prims.size(); ++i)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], (double)prims[i].c[k]);
+                hi[k] = std::max(hi[k], (double)prims[i].c[k]);
+            }
+        double r2 = 0.0;
+        for (int k = 0; k < 3; ++k) out.sph_c[k] = (float)(0.5 * (lo[k] + hi[k]));
+        for (size_t i = n_tri; i < prims.size(); ++i) {
+            double d2 = 0.0;
+            for (int k = 0; k < 3; ++k) d2 += ((double)prims[i].c[k] - out.sph_c[k]) * ((double)prims[i].c[k] - out.sph_c[k]);
+            r2 = std::max(r2, d2);
+        }
+        out.sph_r = (float)(std::sqrt(r2) * (1.0 + 1e-6) + 1e-6);
+    }
+
     Sub s;
     Box rootbox;
-    const uint32_t ref = b.build(s, prims.data(), prims.size(), 0, &rootbox);
+    uint32_t ref;
+    if (n_tri && n_sph) {
+        s.nodes.push_back(BvhNode{});   // root, filled below
+        Box tb, sb;
+        const uint32_t tref = b.build(s, prims.data(), n_tri, 1, &tb);
+        Sub ss;
+        const uint32_t sref_local = b.build(ss, prims.data() + n_tri, n_sph, 1, &sb);
+        out.sph_node0 = (uint32_t)s.nodes.size();
+        const uint32_t sref = Builder::append(s, ss, sref_local);
+        set_child(s.nodes[0], 0, tb, tref);
+        set_child_always(s.nodes[0], 1, sref);
+        s.depth = std::max(s.depth, 1u);
+        ref = 0;
+    } else {
+        ref = b.build(s, prims.data(), prims.size(), 0, &rootbox);
+        if (n_sph) out.sph_node0 = 0;
+    }
     if (ref & kRefLeaf) {
         // Tiny scene: wrap the single leaf in a root node (its leaf sits at depth 1).
         BvhNode root{};
         set_child(root, 0, rootbox, ref);
         // The empty slot's box can never be hit (a point at 1e30); it still
         // names the real leaf so no ref ever points outside the records.
-        set_child(root, 1, empty_box(), ref);
+        set_child_never(root, 1, ref);
         s.nodes.insert(s.nodes.begin(), root);
         s.depth = std::max(s.depth, 1u);
+    }
+    if (out.sph_node0 != 0xFFFFFFFFu) {
+        const float one = 1.0f;
+        uint32_t m;
+        std::memcpy(&m, &one, sizeof m);
+        for (size_t i = out.sph_node0; i < s.nodes.size(); ++i) s.nodes[i].d[2] = m;
     }
     out.nodes = std::move(s.nodes);
     out.leaves = s.leaves;

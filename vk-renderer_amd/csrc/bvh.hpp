@@ -6,12 +6,23 @@
 // from the hits it reports (see trace_kernels.hip).
 //
 // Layout (all records 16-byte aligned for 128-bit loads):
-//   Node (64 B): the bounds of BOTH children plus their refs, so one node
-//   fetch tests two boxes:
-//     a = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
-//     b = (c0.lo.z, c0.hi.z, c1.lo.x, c1.hi.x)
-//     c = (c1.lo.y, c1.hi.y, c1.lo.z, c1.hi.z)
-//     d = (ref0, ref1, 0, 0)
+//   Node (64 B): the bounds of BOTH children, as centre and half-extent per
+//   axis (rounded so the float box contains the exact one), plus their refs,
+//   so one node fetch tests two boxes:
+//     a = (c0.cx, c0.ex, c0.cy, c0.ey)
+//     b = (c0.cz, c0.ez, c1.cx, c1.ex)
+//     c = (c1.cy, c1.ey, c1.cz, c1.ez)
+//     d = (ref0, ref1, m, 0)
+//   m is the float 1.0 in the nodes of the sphere subtree and 0 elsewhere;
+//   the traversal multiplies its per-ray widening by it.
+//   With a centre/extent box the slab test is t = (c - o) / d -+ e / |d|, and
+//   a per-ray widening w of the box is one more term: (e + w) / |d|.
+//   Triangles and spheres live in separate subtrees: when both exist, node 0
+//   is a root whose child 0 is the triangle subtree and child 1 the sphere
+//   subtree (an always-entered box), and the nodes from index sph_node0 on
+//   belong to the sphere subtree.  The traversal widens the boxes of the
+//   sphere subtree per ray (the reference's sphere test is imprecise far from
+//   the sphere, see traverse.hpp).
 //   ref: bit31 = leaf; for a leaf bit30 = sphere leaf, bits 27..29 = count-1,
 //   bits 0..26 = first record; for an inner node the node index.
 //   Triangle record (48 B): (v0, key) (e1, mat_index) (e2, 0) where
@@ -66,6 +77,9 @@ struct BuildOutput {
     std::vector<uint32_t> sph_key; // scan index per sphere record
     uint32_t leaves = 0;
     uint32_t depth = 0;            // deepest leaf (root children are depth 1)
+    uint32_t sph_node0 = 0xFFFFFFFFu;  // first node of the sphere subtree (none: all ones)
+    float sph_c[3] = {0.0f, 0.0f, 0.0f};  // sphere centres lie within sph_r of sph_c
+    float sph_r = 0.0f;
     bool empty = true;
 };
 

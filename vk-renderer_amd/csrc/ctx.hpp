@@ -135,17 +135,17 @@ inline std::vector<uint32_t> emitter_list(const std::vector<vkr_sphere>& s)
     return e;
 }
 
-// 1 when every emitter has finite emission below 2^80 in magnitude and a
-// radius in [2^-40, 2^40]: then its attenuation factor is finite at any hit
-// point that is not within 2^-19 radii of its centre (used by the kernel's
-// exact zero-contribution test).
+// 1 when every emitter has emission below 2^20 in magnitude and a radius in
+// [2^-20, 2^20]: then its attenuation factor is below 2^41 at any hit point
+// at least 2^-9 radii from its centre (used by the kernel's exact
+// zero-contribution tests).
 inline uint32_t emitters_bounded(const std::vector<vkr_sphere>& s, const std::vector<uint32_t>& em)
 {
     for (uint32_t i : em) {
         const vkr_sphere& q = s[i];
         for (int k = 0; k < 3; ++k)
-            if (!(std::fabs(q.mat.emissive[k]) < 0x1p80f)) return 0u;
-        if (!(q.r >= 0x1p-40f && q.r <= 0x1p40f)) return 0u;
+            if (!(std::fabs(q.mat.emissive[k]) < 0x1p20f)) return 0u;
+        if (!(q.r >= 0x1p-20f && q.r <= 0x1p20f)) return 0u;
     }
     return 1u;
 }

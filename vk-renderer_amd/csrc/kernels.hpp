@@ -25,7 +25,8 @@ struct DevScene {
     const uint32_t* emitters;       // scan indices of spheres with emissive != 0
     uint32_t n_tris, n_spheres, n_planes, n_emitters;
     uint32_t n_materials, flags, has_bvh, stack_needed;
-    uint32_t emitters_bounded;      // every emitter: |emissive| < 2^80, 2^-40 <= r <= 2^40
+    uint32_t emitters_bounded;      // every emitter: |emissive| < 2^20, 2^-20 <= r <= 2^20
+    float sph_c[3], sph_r;          // every sphere centre lies within sph_r of sph_c
 };
 
 // One frame of work (a single trace call, or one job of a batch).

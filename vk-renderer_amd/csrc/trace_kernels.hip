@@ -51,6 +51,8 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
     uint32_t c_rays = 0, c_shadow = 0, c_traced = 0, wave_steps = 0;
     bool new_path = false;
     bool nee_zero = false;              // every emitter of this hit contributes exactly +-0
+    bool nee_fin = false;               // the BRDF factor of this hit is finite (bounded) for any sample
+    bool nee_mirror = false;            // metalness 1, roughness 0: zero unless (N.H)^2 rounds to 1
     WorkCnt wc{};
 
     // Claims pool index g for this lane (no-op for indices outside the image).
@@ -161,25 +163,100 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
                 const f3 alb = ld3(hm->albedo);
                 const float rough = hm->roughness, metal = hm->metalness;
                 const f3 V = normalize(sub(ld3(j.cam_pos), P));
-                if (nee_zero) {
-                    // Such emitters are counted and their two random draws
-                    // (:464-465) skipped by advancing the stream state alone.
-                    // A hit point within 2^-19 radii of an emitter's centre
-                    // (attenuation may overflow) is evaluated in full below.
-                    while (ei < j.sc.n_emitters) {
-                        const vkr_sphere* S = j.sc.spheres + j.sc.emitters[ei];
-                        const f3 l0 = sub(ld3(S->P), P);
-                        const float sr = S->r;
-                        if (!(dot(l0, l0) > (sr * sr) * 0x1p-38f)) break;
-                        salt = salt + 1.0f;
-                        salt = salt + 1.0f;
-                        ++rays_px;
-                        ++c_rays;
-                        ++c_shadow;
-                        ++ei;
-                    }
-                }
                 while (ei < j.sc.n_emitters) {
+                    if (nee_zero || nee_fin || nee_mirror) {
+                        // Emitters certified to contribute exactly +-0 before
+                        // sampling are counted, and their two random draws
+                        // (:464-465) skipped by advancing the stream state
+                        // alone.  The tests carry margins far above the
+                        // rounding of the fast reciprocal / square-root
+                        // estimates they use; any emitter they cannot certify
+                        // is evaluated in full below.
+                        const float nn = dot(N, N);
+                        while (ei < j.sc.n_emitters) {
+                            const vkr_sphere* S = j.sc.spheres + j.sc.emitters[ei];
+                            const f3 l0 = sub(ld3(S->P), P);
+                            const float sr = S->r;
+                            const float ll2 = dot(l0, l0);
+                            // Attenuation < 2^41: the hit point is at least 2^-9
+                            // radii from the centre.
+                            if (!(ll2 >= (sr * sr) * 0x1p-18f)) break;
+                            // A sample L has cosa >= cos_a_max and sina <= sig,
+                            // sig^2 = s2 + 1e-6 >= the kernel's 1 - cos_a_max^2, so
+                            // it lies within t = 1.5708 sig of w = l0 / |l0|.
+                            const float il = __builtin_amdgcn_rsqf(ll2);
+                            const float sw = sr * il;
+                            const float sig2 = fminf(sw * sw + 1e-6f, 1.0f);
+                            const float sig = __builtin_amdgcn_sqrtf(sig2) * (1.0f + 1e-6f);
+                            const float t = 1.5708f * sig;
+                            // Every certificate needs the Fresnel term finite: with
+                            // V at least 0.01 rad outside the cone, dot(H, V) <= 1 - 1e-5,
+                            // so pow(1 - dot(H, V), 5) never sees a negative base
+                            // (NaN, :276).
+                            const f3 w = muls(l0, il);
+                            const float cph = dot(V, w);
+                            const float tv = t + 0.01f;
+                            if (!(tv < 1.5f && cph < 1.0f - 0.5f * tv * tv - 1e-6f)) break;
+                            bool zero = nee_zero;
+                            if (!zero && nee_fin) {
+                                // (b) cone below the horizon: with c = cos(N, l0)
+                                // < 0 and s = sin(N, l0) (from |N x l0|, no
+                                // cancellation), cos(N, L) <= sig s + cl c < -1e-4.
+                                const float inv = __builtin_amdgcn_rsqf(nn * ll2);
+                                const float c = dot(N, l0) * inv;
+                                const f3 cr = cross(N, l0);
+                                const float s = __builtin_amdgcn_sqrtf(dot(cr, cr)) * inv;
+                                const float cl = __builtin_amdgcn_sqrtf(1.0f - sig2) * (1.0f - 1e-6f);
+                                zero = c < 0.0f && sig * s + cl * c < -1e-4f;
+                            }
+                            if (!zero && nee_mirror) {
+                                // (c) mirror hits: H = normalize(V + L) stays within
+                                // dl >= t / (2 cos((phi0 + t) / 2)) of H0 = normalize(V + w)
+                                // (|dH| <= |dL| / |V + L|; phi0 = angle(V, w) < 107.5 deg,
+                                // t < 1, so the path never nears V + L = 0).  If
+                                // |N| cos(N, H) then stays at least 2e-4 away from 1,
+                                // NdotH^2 cannot round to 1 and the sample is +-0.
+                                const f3 vw = cross(V, w);
+                                const float sph = __builtin_amdgcn_sqrtf(dot(vw, vw));
+                                const float csum = (cph >= 0.0f ? cph * (1.0f - 0.5f * t * t) : cph) - sph * t;
+                                const float half = __builtin_amdgcn_sqrtf(fmaxf(0.5f * (1.0f + csum), 0.0f));
+                                if (cph > -0.3f && half >= 0.05f && t < 1.0f) {
+                                    const float dl = t * __builtin_amdgcn_rcpf(2.0f * half) * 1.001f + 1e-5f;
+                                    const f3 h = add(V, w);
+                                    const float hh = dot(h, h);
+                                    const float ihn = __builtin_amdgcn_rsqf(hh * nn);
+                                    const float c0 = dot(N, h) * ihn;
+                                    const f3 nh = cross(N, h);
+                                    const float s0 = __builtin_amdgcn_sqrtf(dot(nh, nh)) * ihn;
+                                    const float nrm = __builtin_amdgcn_sqrtf(nn);
+                                    const float cd = 1.0f - 0.5f * dl * dl;
+                                    const float hi =
+                                        c0 >= cd - 1e-4f ? 1.0f : (c0 >= 0.0f ? c0 : c0 * cd) + s0 * dl;
+                                    const float lo = (c0 >= 0.0f ? c0 * cd : c0) - s0 * dl;
+                                    zero = dl < 1.0f && hh > 1e-6f &&
+                                           (nrm * hi < 1.0f - 2e-4f || nrm * lo > 1.0f + 2e-4f);
+                                }
+                            }
+                            if (!zero && nee_fin) {
+                                // (a) attenuation exactly 0 (:478-479): with the
+                                // kernel's own x, emis / x^2 <= kAttSub in every
+                                // channel, tested with a 2^-16 margin.
+                                const float tl = length(l0) - sr;
+                                const float x = tl / sr + 1.0f;
+                                const float lim = (kAttSub * (x * x)) * (1.0f - 0x1p-16f);
+                                const f3 em = ld3(S->mat.emissive);
+                                zero = em.x <= lim && em.y <= lim && em.z <= lim;
+                            }
+                            if (!zero) break;
+                            salt = salt + 1.0f;      // the sample's two draws (:464-465)
+                            salt = salt + 1.0f;
+                            ++rays_px;
+                            ++c_rays;
+                            ++c_shadow;
+                            ++ei;
+                        }
+                        if (ei >= j.sc.n_emitters) break;
+                    }
                     const vkr_sphere* S = j.sc.spheres + j.sc.emitters[ei];
                     const f3 sP = ld3(S->P);
                     const float sr = S->r;
@@ -264,13 +341,39 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
                     // Metalness 1 and roughness 0 (the reference's `mirror`):
                     // kD = (1 - F) * 0 and NDF = 0 / (pi * denom^2), where
                     // denom = 1 - NdotH^2 >= 8e-5 once |N|^2 < 0.9999 (|H| = 1),
-                    // so with finite albedo, F, G and attenuation every
-                    // emitter's contribution is exactly +-0 whatever its sample.
+                    // so with finite albedo, F, G and attenuation an emitter's
+                    // contribution is exactly +-0 whatever its sample (F is
+                    // finite when V stays outside the emitter's cone, which is
+                    // checked per emitter).
+                    //
+                    // More generally, with |albedo|, |metalness| <= 16,
+                    // k = (r+1)^2/8 in [2^-10, 1], |N|^2 <= 2^10 and the GGX
+                    // denominator 1 - NdotH^2 (1 - a2) >= 2^-10 for every
+                    // |H| = 1 (or a2 >= 1), the factor (kD alb/pi + specular)
+                    // is below 2^72 for any sample: F <= 2^10, G <= 2^30,
+                    // NDF <= 2^22.  An emitter whose attenuation is exactly 0,
+                    // or whose whole sampling cone lies below the horizon
+                    // (NdotL = +0), then contributes exactly +-0 as well.
                     {
                         const f3 alb = ld3(m->albedo);
-                        nee_zero = j.sc.emitters_bounded && m->metalness == 1.0f && m->roughness == 0.0f &&
+                        const float nn = dot(N, N);
+                        const float rough = m->roughness, metal = m->metalness;
+                        nee_zero = j.sc.emitters_bounded && metal == 1.0f && rough == 0.0f &&
                                    fabsf(alb.x) < 1e30f && fabsf(alb.y) < 1e30f && fabsf(alb.z) < 1e30f &&
-                                   dot(N, N) < 0.9999f;
+                                   nn < 0.9999f;
+                        const float rp = rough + 1.0f;
+                        const float k = (rp * rp) / 8.0f;
+                        const float a = rough * rough;
+                        const float a2 = a * a;
+                        nee_fin = j.sc.emitters_bounded && fabsf(alb.x) <= 16.0f && fabsf(alb.y) <= 16.0f &&
+                                  fabsf(alb.z) <= 16.0f && fabsf(metal) <= 16.0f && k >= 0x1p-10f && k <= 1.0f &&
+                                  nn <= 1024.0f &&
+                                  (a2 >= 1.0f || 1.0f - (nn * (1.0f + 0x1p-16f)) * (1.0f - a2) >= 0x1p-10f);
+                        // Other mirror hits: kD = 0 and NDF = 0 / (pi (1 - NdotH^2)^2)
+                        // is 0 unless NdotH^2 rounds to exactly 1 (then NaN).
+                        nee_mirror = !nee_zero && j.sc.emitters_bounded && metal == 1.0f && rough == 0.0f &&
+                                     fabsf(alb.x) <= 16.0f && fabsf(alb.y) <= 16.0f && fabsf(alb.z) <= 16.0f &&
+                                     nn <= 1024.0f;
                     }
                     if (j.sc.n_emitters > 0) phase = kPhNee;
                     else fin_diffuse = true;

@@ -87,6 +87,15 @@ __device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, floa
         const float iy = 1.0f / (fabsf(d.y) > 1e-20f ? d.y : copysignf(1e-20f, d.y));
         const float iz = 1.0f / (fabsf(d.z) > 1e-20f ? d.z : copysignf(1e-20f, d.z));
         const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
+        const float ax = fabsf(ix), ay = fabsf(iy), az = fabsf(iz);
+        // The sphere test (sphere_t, Assets/Tracer.comp:314-330) rounds its
+        // discriminant with an error up to ~60 ulp of |o - c|^2, so from far
+        // away it reports hits up to ~1.4e-3 |o - c| outside the sphere.
+        // The sphere subtree's boxes are widened by 2e-3 (|o - C| + R), which
+        // bounds that distance for every centre c within R of C.
+        const f3 oc = sub(o, f3{sc.sph_c[0], sc.sph_c[1], sc.sph_c[2]});
+        const float pad = 2e-3f * (sqrtf(dot(oc, oc)) + sc.sph_r);
+        const float px = fminf(pad * ax, 3e38f), py = fminf(pad * ay, 3e38f), pz = fminf(pad * az, 3e38f);
         uint32_t ref = 0;
         int sp = 0;
         while (true) {
@@ -107,16 +116,20 @@ __device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, floa
                         count_wave(wc.wave_nodes);
                     }
                     const float cb = cull + fabsf(cull) * 1e-5f + 1e-5f;
-                    const float a0 = __builtin_fmaf(A.x, ix, -ox), a1 = __builtin_fmaf(A.y, ix, -ox);
-                    const float a2 = __builtin_fmaf(A.z, iy, -oy), a3 = __builtin_fmaf(A.w, iy, -oy);
-                    const float a4 = __builtin_fmaf(B.x, iz, -oz), a5 = __builtin_fmaf(B.y, iz, -oz);
-                    const float b0 = __builtin_fmaf(B.z, ix, -ox), b1 = __builtin_fmaf(B.w, ix, -ox);
-                    const float b2 = __builtin_fmaf(C.x, iy, -oy), b3 = __builtin_fmaf(C.y, iy, -oy);
-                    const float b4 = __builtin_fmaf(C.z, iz, -oz), b5 = __builtin_fmaf(C.w, iz, -oz);
-                    const float n0 = fmaxf(fmaxf(fminf(a0, a1), fminf(a2, a3)), fmaxf(fminf(a4, a5), 0.0f));
-                    const float f0 = fminf(fminf(fmaxf(a0, a1), fmaxf(a2, a3)), fminf(fmaxf(a4, a5), cb));
-                    const float n1 = fmaxf(fmaxf(fminf(b0, b1), fminf(b2, b3)), fmaxf(fminf(b4, b5), 0.0f));
-                    const float f1 = fminf(fminf(fmaxf(b0, b1), fmaxf(b2, b3)), fminf(fmaxf(b4, b5), cb));
+                    // Slab test on centre/half-extent boxes: t = (c - o) i -+ (e + w) |i|,
+                    // with the widening w only in the sphere subtree (m = 1 there).
+                    const float m = __uint_as_float(D.z);
+                    const float qx = m * px, qy = m * py, qz = m * pz;
+                    const float a0 = __builtin_fmaf(A.x, ix, -ox), a1 = __builtin_fmaf(A.y, ax, qx);
+                    const float a2 = __builtin_fmaf(A.z, iy, -oy), a3 = __builtin_fmaf(A.w, ay, qy);
+                    const float a4 = __builtin_fmaf(B.x, iz, -oz), a5 = __builtin_fmaf(B.y, az, qz);
+                    const float b0 = __builtin_fmaf(B.z, ix, -ox), b1 = __builtin_fmaf(B.w, ax, qx);
+                    const float b2 = __builtin_fmaf(C.x, iy, -oy), b3 = __builtin_fmaf(C.y, ay, qy);
+                    const float b4 = __builtin_fmaf(C.z, iz, -oz), b5 = __builtin_fmaf(C.w, az, qz);
+                    const float n0 = fmaxf(fmaxf(a0 - a1, a2 - a3), fmaxf(a4 - a5, 0.0f));
+                    const float f0 = fminf(fminf(a0 + a1, a2 + a3), fminf(a4 + a5, cb));
+                    const float n1 = fmaxf(fmaxf(b0 - b1, b2 - b3), fmaxf(b4 - b5, 0.0f));
+                    const float f1 = fminf(fminf(b0 + b1, b2 + b3), fminf(b4 + b5, cb));
                     const bool h0 = n0 <= f0, h1 = n1 <= f1;
                     // Branch-free step: the far child is always written to
                     // the free slot above the top (sp < STACK at any inner

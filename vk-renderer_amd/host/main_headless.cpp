@@ -3,7 +3,10 @@
 // print the frame rate, and write the presented image.
 //
 //   vkr_headless [--frames N] [--res R] [--present WxH] [--seeds a,b,c]
-//                [--out image.ppm] [--raw prefix]
+//                [--poses file] [--out image.ppm] [--raw prefix]
+// --poses replays recorded camera input in place of Main.cpp:165-190's
+// keyboard/mouse handling: one "x y z pitch yaw" line per frame (cycled;
+// '#' starts a comment), each applied through Camera::update.
 // --raw writes prefix.traced0.rgba, prefix.traced1.rgba, prefix.present.rgba
 // and prefix.seeds.txt for checking against the CPU restatement.
 #include <chrono>
@@ -17,6 +20,24 @@
 #include "GraphicsDevice.h"
 
 namespace {
+struct Pose {
+    float x, y, z, pitch, yaw;
+};
+
+bool read_poses(const std::string& path, std::vector<Pose>& out)
+{
+    FILE* f = std::fopen(path.c_str(), "r");
+    if (!f) return false;
+    char line[512];
+    while (std::fgets(line, sizeof(line), f)) {
+        if (char* c = std::strchr(line, '#')) *c = 0;
+        Pose p{};
+        if (std::sscanf(line, "%f %f %f %f %f", &p.x, &p.y, &p.z, &p.pitch, &p.yaw) == 5) out.push_back(p);
+    }
+    std::fclose(f);
+    return !out.empty();
+}
+
 bool write_file(const std::string& path, const void* data, size_t n)
 {
     FILE* f = std::fopen(path.c_str(), "wb");
@@ -32,7 +53,7 @@ int main(int argc, char** argv)
     int frames = 60;
     unsigned res = 1024, pw = 1024, ph = 768;
     std::vector<float> seeds;
-    std::string out, raw;
+    std::string out, raw, poses_path;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         auto next = [&]() { return i + 1 < argc ? std::string(argv[++i]) : std::string(); };
@@ -41,6 +62,7 @@ int main(int argc, char** argv)
         else if (a == "--present") std::sscanf(next().c_str(), "%ux%u", &pw, &ph);
         else if (a == "--out") out = next();
         else if (a == "--raw") raw = next();
+        else if (a == "--poses") poses_path = next();
         else if (a == "--seeds") {
             const std::string s = next();
             size_t p = 0;
@@ -57,6 +79,11 @@ int main(int argc, char** argv)
     }
     if (frames < 1 || res == 0 || res > 16384 || pw == 0 || ph == 0) {
         std::fprintf(stderr, "bad arguments\n");
+        return 2;
+    }
+    std::vector<Pose> poses;
+    if (!poses_path.empty() && !read_poses(poses_path, poses)) {
+        std::fprintf(stderr, "cannot read poses from %s\n", poses_path.c_str());
         return 2;
     }
 
@@ -81,6 +108,13 @@ int main(int argc, char** argv)
     std::vector<float> used;
     const auto t0 = std::chrono::steady_clock::now();
     for (int f = 0; f < frames; ++f) {
+        if (!poses.empty()) {
+            const Pose& q = poses[(size_t)f % poses.size()];
+            camera.data.pos = vec3{q.x, q.y, q.z};
+            camera.aux.pitch = q.pitch;
+            camera.aux.yaw = q.yaw;
+            camera.update();
+        }
         frame_data.camera = camera.data;
         device.Draw(frame_data);
         used.push_back(device.LastSeed());

@@ -189,6 +189,28 @@ def soup100k(seed: int = 0, n_tris: int = 100_000, width: int = 1920, height: in
                     width, height, spp, max_depth)
 
 
+def far_sphere(res: int = 128, dist: float = 2000.0, span: float = 1.5e-3) -> Workload:
+    """A unit sphere seen from `dist` away through a narrow camera (the image
+    spans +-span*dist around it), plus a mirror-triangle soup and two more
+    spheres off screen.  From that far the reference's sphere test
+    (Assets/Tracer.comp:314-330) rounds its discriminant h = b^2 - 4c with an
+    error of tens of ulp of |o - c|^2, so it reports hits on rays that pass
+    up to ~0.2 outside the sphere: a traversal that culls by the exact box
+    loses them."""
+    rng = rng_for(7, 0)
+    M = reference_materials()
+    sph = spheres_from([M["matte_white"], M["plastic"], M["light"]],
+                       [(0.3, -0.2, 0.1), (40.0, 5.0, 30.0), (0.0, 300.0, -1500.0)], [1.0, 2.0, 40.0])
+    tris = random_soup(rng, 64, lo=-60.0, hi=-20.0)
+    cam = np.zeros((), vkr.CAMERA)
+    cam["pos"] = (0.0, 0.0, -dist)
+    cam["dir"] = (0.0, 0.0, 1.0)
+    cam["right"] = (span, 0.0, 0.0)
+    cam["up"] = (0.0, span, 0.0)
+    sc = vkr.Scene(tris=tris, spheres=sph, tri_material=M["mirror"])
+    return Workload(f"far_sphere_{res}", sc, frame(cam, 1.0), res, res, 1, 2)
+
+
 def value_noise_fbm(rng, x, z, octaves=6, lacunarity=2.0, gain=0.5, base_freq=1.0 / 128.0, lattice=512):
     table = rng.uniform(-1.0, 1.0, (lattice, lattice))
     total = np.zeros_like(x)
