@@ -296,6 +296,9 @@ vkr_error vkr_trace_frame(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd
         const uint32_t grid = persistent ? std::min(persistent_grid(block, stack, count, false, ctx->device),
                                                     (a.total + block - 1u) / block)
                                          : (a.total + block - 1u) / block;
+        const vkr_error le = ensure_lanes(ctx, (size_t)grid * block);
+        if (le != VKR_SUCCESS) return le;
+        a.lanes = ctx->lanes.as<char>();
         VKR_HIP(hipEventRecord(ctx->ev0, st));
         VKR_HIP(launch_path(a, block, stack, count, false, grid, st));
         VKR_HIP(hipEventRecord(ctx->ev1, st));

@@ -119,11 +119,15 @@ vkr_error vkr_trace_batch(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n_jo
     uint32_t* pool = ctx->pool.as<uint32_t>();
     const uint32_t blk = block ? block : 128u;
     const bool count = (ctx->flags & VKR_CTX_COUNT_WORK) != 0;
-    const PathLaunch a = batch_launch(plan, ctx->jobs.as<JobDesc>(), ctx->job_start.as<uint32_t>(), n_jobs, chunk,
-                                      pool, cnt);
+    PathLaunch a = batch_launch(plan, ctx->jobs.as<JobDesc>(), ctx->job_start.as<uint32_t>(), n_jobs, chunk, pool,
+                                cnt);
+    const uint32_t grid = batch_grid(plan, blk, count, ctx->device);
+    const vkr_error le = ensure_lanes(ctx, (size_t)grid * blk);
+    if (le != VKR_SUCCESS) return le;
+    a.lanes = ctx->lanes.as<char>();
     VKR_HIP(launch_zero_counters(cnt, pool, st));
     VKR_HIP(hipEventRecord(ctx->ev0, st));
-    VKR_HIP(launch_path(a, blk, plan.stack, count, true, batch_grid(plan, blk, count, ctx->device), st));
+    VKR_HIP(launch_path(a, blk, plan.stack, count, true, grid, st));
     VKR_HIP(hipEventRecord(ctx->ev1, st));
     ctx->timed = true;
     ctx->last_counters = nullptr;
@@ -152,9 +156,11 @@ vkr_error vkr_batch_graph_create(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32
     VKR_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
     const uint32_t blk = block ? block : 128u;
     const bool count = (ctx->flags & VKR_CTX_COUNT_WORK) != 0;
-    const PathLaunch a = batch_launch(plan, g->jobs.as<JobDesc>(), g->job_start.as<uint32_t>(), n_jobs, chunk,
-                                      g->pool.as<uint32_t>(), g->counters.as<DevCounters>());
+    PathLaunch a = batch_launch(plan, g->jobs.as<JobDesc>(), g->job_start.as<uint32_t>(), n_jobs, chunk,
+                                g->pool.as<uint32_t>(), g->counters.as<DevCounters>());
     const uint32_t grid = batch_grid(plan, blk, count, ctx->device);
+    VKR_HIP(g->lanes.alloc((size_t)grid * blk * kLaneBytes));   // the graph's own records
+    a.lanes = g->lanes.as<char>();
     hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
         const hipError_t e1 = launch_zero_counters(g->counters.as<DevCounters>(), g->pool.as<uint32_t>(), cap);

@@ -83,7 +83,7 @@ struct Scene {
 struct Graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    DevBuf jobs, job_start, pool, counters;
+    DevBuf jobs, job_start, pool, counters, lanes;
     uint64_t total = 0;
     ~Graph()
     {
@@ -108,6 +108,7 @@ struct vkr_ctx {
     std::vector<std::unique_ptr<vkr::Scene>> scenes;
     std::vector<std::unique_ptr<vkr::Graph>> graphs;
     vkr::DevBuf counters, pool, jobs, job_start;
+    vkr::DevBuf lanes;                 // per-lane ray records of direct launches (grown on demand)
     void* host_jobs = nullptr;         // pinned staging for batched job tables
     size_t host_jobs_bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_copy = nullptr;
@@ -209,6 +210,18 @@ inline vkr_error ensure_events(vkr_ctx* ctx)
     if (!ctx->ev0) VKR_HIP(hipEventCreate(&ctx->ev0));
     if (!ctx->ev1) VKR_HIP(hipEventCreate(&ctx->ev1));
     if (!ctx->ev_copy) VKR_HIP(hipEventCreateWithFlags(&ctx->ev_copy, hipEventDisableTiming));
+    return VKR_SUCCESS;
+}
+
+// Makes the context's per-lane ray records hold `lanes` lanes.  Growing
+// waits for the device first: a launch in flight may still use the old
+// buffer.
+inline vkr_error ensure_lanes(vkr_ctx* ctx, size_t lanes)
+{
+    const size_t bytes = lanes * kLaneBytes;
+    if (ctx->lanes.bytes >= bytes) return VKR_SUCCESS;
+    VKR_HIP(hipDeviceSynchronize());
+    VKR_HIP(ctx->lanes.alloc(bytes));
     return VKR_SUCCESS;
 }
 

@@ -62,6 +62,18 @@ struct DevCounters {
 // 1 refills too often).
 constexpr uint32_t kDefaultChunk = 8u;
 
+// Per-lane ray record of the path kernel (one per lane of a launch, in
+// global memory):
+//   [0, 64)    path state parked while the wave traces
+//   [64, 80)   this step's ray origin xyz, w = 1.0 for shadow rays, 0.0 for a
+//              closest-hit ray
+//   [80, ...)  kRaySlots slots of 32 bytes: (d xyz, t_max), (next-event
+//              contribution xyz, 0)
+constexpr uint32_t kRaySlots = 8;
+constexpr uint32_t kRecHead = 64u;
+constexpr uint32_t kRecSlots = 80u;
+constexpr uint32_t kLaneBytes = kRecSlots + 32u * kRaySlots;
+
 struct PathLaunch {
     JobDesc job;                  // single-frame launches
     const JobDesc* jobs;          // batched launches
@@ -72,6 +84,7 @@ struct PathLaunch {
     uint32_t chunk;               // refill when at least this many lanes idle
     uint32_t* pool;               // persistent pool cursor (device)
     DevCounters* counters;
+    char* lanes;                  // per-lane ray records, grid x block x kLaneBytes (traverse.hpp)
 };
 
 // Adjoint launch (path replay), see adjoint.hip.

@@ -3,9 +3,12 @@
 // The reference (Assets/Tracer.comp) runs one invocation per pixel that
 // loops SAMPLES paths of up to DEPTH bounces and scans every primitive
 // linearly per ray.  Here:
-//   * each lane runs one pixel as a small state machine; every loop
-//     iteration traces exactly one ray per lane (a path segment or one
-//     next-event shadow ray), so all lanes of a wave traverse together;
+//   * each lane runs one pixel as a small state machine; in every loop
+//     iteration a lane queues either its path segment or the shadow rays of
+//     up to kRaySlots emitters (with their contributions, evaluated first),
+//     and the wave traces all queued rays from a shared pool: a lane whose
+//     ray ends takes the next one, so the wave waits for its total work, not
+//     for its longest lane;
 //   * persistent waves refill idle lanes from a global pixel pool with one
 //     wave-wide ballot + one atomic per refill (8x8 tiles in pool order);
 //   * the linear scan is replaced by a BVH (two child boxes per 64-byte node,
@@ -26,15 +29,22 @@
 namespace vkr {
 
 // The path program of Assets/Tracer.comp:433-593 as a per-lane state machine.
-// Each loop iteration: (1) refill idle lanes, (2) set up this lane's ray
-// (a path segment, or the shadow ray of the next emitter), (3) traverse,
-// (4) shade, then run the shared tail blocks (finish diffuse -> Russian
-// roulette -> end of path -> end of pixel) at most once each.
+// Each loop iteration: (1) refill idle lanes, (2) queue this lane's rays
+// (a path segment, or the shadow rays of the next emitters), (3) trace the
+// wave's rays together, (4) shade, then run the shared tail blocks (finish
+// diffuse -> Russian roulette -> end of path -> end of pixel) at most once
+// each.
 template <int BLOCK, int STACK, bool COUNT, bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
 {
     __shared__ uint32_t s_stack[STACK * BLOCK];
+    __shared__ uint16_t s_pool[BLOCK * kRaySlots];   // per wave: the step's rays, (lane << 4) | slot
+    __shared__ uint32_t s_occ[BLOCK];                // per lane: occluded shadow rays (bit per slot)
+    __shared__ uint2 s_res[BLOCK];                   // per lane: closest hit (prim, t)
     uint32_t* stk = s_stack + threadIdx.x;
+    const uint32_t wbase = __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
+    char* const wrec = A.lanes + ((size_t)blockIdx.x * BLOCK + wbase) * kLaneBytes;
+    char* const rec = wrec + __lane_id() * kLaneBytes;
 
     // Lane state.
     bool has = false;
@@ -43,7 +53,7 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
     float uvx = 0.0f, uvy = 0.0f, salt = 0.0f;
     uint32_t smp = 0, depth = 0, phase = kPhTrace;
     f3 ro{}, rd{}, acc{}, mask{}, accum{};
-    f3 P{}, N{}, dj{}, e{}, cnee{};
+    f3 P{}, N{}, dj{}, e{};
     const vkr_material* hm = nullptr;   // material of the current diffuse hit
     uint32_t ei = 0;
     uint32_t rays_px = 0, prim0 = kMiss, drec = 0, drep = 0;
@@ -138,28 +148,27 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
                 drep = j.decisions[((size_t)py * j.W + px) * j.spp + smp];
         }
 
-        f3 o{}, d{};
-        float tmax = 0.0f;
-        bool any = false, need = false;
+        uint32_t nray = 0;   // rays this lane queues for the step
         if (has) {
             if (phase == kPhTrace) {
                 acc = clamps(acc, 0.0f, 1.0f);                      // :441
                 const float dp1 = (float)(depth + 1u);
-                tmax = 3000.0f / (dp1 * dp1);                        // :444
-                o = ro;
-                d = rd;
-                need = true;
+                const float tmax = 3000.0f / (dp1 * dp1);            // :444
+                *reinterpret_cast<float4*>(rec + kRecHead) = make_float4(ro.x, ro.y, ro.z, 0.0f);
+                *reinterpret_cast<float4*>(rec + kRecSlots) = make_float4(rd.x, rd.y, rd.z, tmax);
+                nray = 1;
                 ++rays_px;
                 ++c_rays;
                 ++c_traced;
             } else {
                 // Next-event samples (:464-469) and their unoccluded
-                // contributions (:475-502), evaluated before the shadow ray.
+                // contributions (:475-502), evaluated before the shadow rays.
                 // A contribution of exactly +-0 cannot change e whatever the
-                // ray's outcome, so that ray is counted but not traversed and
-                // the lane moves straight on to the next emitter (same RNG
-                // order); the step ends at the first ray that must be traced,
-                // or with the emitter list done.
+                // ray's outcome, so that ray is counted but not traversed.
+                // Rays that must be traced are queued with their
+                // contributions, up to kRaySlots per step, and added to e in
+                // emitter order once their outcomes are known (the same
+                // additions in the same order as the reference's loop).
                 const f3 alb = ld3(hm->albedo);
                 const float rough = hm->roughness, metal = hm->metalness;
                 const f3 V = normalize(sub(ld3(j.cam_pos), P));
@@ -282,27 +291,67 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
                     const float denominator = 4.0f * gmax(dot(N, V), 0.0f) * gmax(dot(N, L), 0.0f);
                     const f3 specular = divs(numerator, gmax(denominator, 0.001f));
                     const float NdotL = gmax(dot(N, L), 0.0f);
-                    cnee = muls(mul(add(divs(mul(kD, alb), kPi), specular), att), NdotL);
+                    const f3 cnee = muls(mul(add(divs(mul(kD, alb), kPi), specular), att), NdotL);
                     ++rays_px;
                     ++c_rays;
                     ++c_shadow;
-                    need = !(cnee.x == 0.0f && cnee.y == 0.0f && cnee.z == 0.0f);
-                    if (need) {
+                    ++ei;
+                    if (!(cnee.x == 0.0f && cnee.y == 0.0f && cnee.z == 0.0f)) {
                         ++c_traced;
-                        o = P;
-                        d = L;
-                        tmax = tl;
-                        any = true;
-                        break;
+                        float4* sl = reinterpret_cast<float4*>(rec + kRecSlots + nray * 32u);
+                        sl[0] = make_float4(L.x, L.y, L.z, tl);
+                        sl[1] = make_float4(cnee.x, cnee.y, cnee.z, 0.0f);
+                        if (++nray >= kRaySlots) break;
                     }
-                    if (++ei >= j.sc.n_emitters) break;
                 }
+                *reinterpret_cast<float4*>(rec + kRecHead) = make_float4(P.x, P.y, P.z, 1.0f);
             }
         }
 
-        TravOut tr{kMiss, 0.0f, false};
-        if (has && need) tr = traverse<BLOCK, COUNT>(j.sc, o, d, tmax, j.eps, any, stk, wc);
+        // Queue the step's rays in the wave's pool and trace them together.
+        {
+            const unsigned long long lt = (1ull << __lane_id()) - 1ull;
+            uint32_t pre = 0, tot = 0;
+            for (uint32_t b = 0; b < 4u; ++b) {   // nray <= kRaySlots = 8
+                const unsigned long long m = __ballot((nray >> b) & 1u);
+                pre += (uint32_t)__popcll(m & lt) << b;
+                tot += (uint32_t)__popcll(m) << b;
+            }
+            s_occ[threadIdx.x] = 0u;
+            for (uint32_t k = 0; k < nray; ++k)
+                s_pool[wbase * kRaySlots + pre + k] = (uint16_t)((__lane_id() << 4) | k);
+            // Path state that tracing does not need waits in the record, so it
+            // holds no registers meanwhile.
+            float4* park = reinterpret_cast<float4*>(rec);
+            park[0] = make_float4(ro.x, ro.y, ro.z, dj.x);
+            park[1] = make_float4(rd.x, rd.y, rd.z, dj.y);
+            park[2] = make_float4(P.x, P.y, P.z, dj.z);
+            park[3] = make_float4(N.x, N.y, N.z, 0.0f);
+            // The pool, the records and the result words are shared by the
+            // wave's lanes: order the writes above before the reads below.
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            traverse_pool<BLOCK, COUNT>(j.sc, j.eps, tot, s_pool + wbase * kRaySlots, s_occ + wbase, s_res + wbase,
+                                        wrec, stk, wc);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const float4 k0 = park[0], k1 = park[1], k2 = park[2], k3 = park[3];
+            ro = f3{k0.x, k0.y, k0.z};
+            rd = f3{k1.x, k1.y, k1.z};
+            P = f3{k2.x, k2.y, k2.z};
+            N = f3{k3.x, k3.y, k3.z};
+            dj = f3{k0.w, k1.w, k2.w};
+        }
         if (!has) continue;
+        TravOut tr{kMiss, 0.0f, false};
+        if (phase == kPhTrace) {
+            const uint2 r = s_res[threadIdx.x];
+            tr.prim = r.x;
+            tr.t = __uint_as_float(r.y);
+            tr.hit = r.x != kMiss;
+        }
 
         bool fin_diffuse = false, do_rr = false, path_end = false;
         if (phase == kPhTrace) {
@@ -400,9 +449,11 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
                 }
             }
         } else {
-            if (need) {
-                if (!tr.hit) e = add(e, cnee);                       // :473, :502
-                ++ei;
+            const uint32_t occ = s_occ[threadIdx.x];
+            for (uint32_t k = 0; k < nray; ++k) {
+                if ((occ >> k) & 1u) continue;                       // :473
+                const float4 c = reinterpret_cast<const float4*>(rec + kRecSlots + k * 32u)[1];
+                e = add(e, f3{c.x, c.y, c.z});                       // :502
             }
             if (ei >= j.sc.n_emitters) fin_diffuse = true;
         }

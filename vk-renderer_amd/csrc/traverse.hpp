@@ -21,13 +21,13 @@ constexpr uint32_t kDone = 0xFFFFFFFFu;
 
 
 // A wave leaves the node loop for the leaf pass once at least kLeafExit8/8
-// of its lanes that are still traversing wait at a leaf; the others keep
-// their place in the tree and go on in the next round.  Every lane still
-// visits the same nodes and primitives in the same order (results do not
-// depend on it); only the grouping of lanes into wave iterations changes.
-// 4/8 measured best on incoherent 1080p / 100k-triangle frames with leaves
-// of at most 2 primitives (8/8 is the plain while-while loop: node-loop SIMD
-// efficiency 14% -> 31%).
+// of its lanes that are still traversing wait at a leaf (or, in the pooled
+// loop, for a new ray); the others keep their place in the tree and go on in
+// the next round.  Every ray still visits the same nodes and primitives in
+// the same order (results do not depend on it); only the grouping of lanes
+// into wave iterations changes.  4/8 measured best on incoherent 1080p /
+// 100k-triangle frames with leaves of at most 2 primitives (2/8 and 6/8 are
+// 12% and 4% slower with the pooled loop).
 constexpr uint32_t kLeafExit8 = 4;
 constexpr uint32_t kPhTrace = 0;
 constexpr uint32_t kPhNee = 1;
@@ -59,210 +59,308 @@ __device__ __forceinline__ void cswap(float& ta, uint32_t& ka, float& tb, uint32
     ta = t0; tb = t1; ka = k0; kb = k1;
 }
 
-// trace_ray of Assets/Tracer.comp:374-431 over the BVH.
+// One ray in flight: trace_ray of Assets/Tracer.comp:374-431 over the BVH.
 //   any == false: closest hit with the reference's acceptance rule.
 //   any == true : only "was anything accepted" (shadow rays, :473).
-template <int BLOCK, bool COUNT>
-__device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, float tmax, float eps, bool any,
-                                            uint32_t* stk, WorkCnt& wc)
-{
-    const float tmax_eps = tmax + eps;
-    const float win = 4.0f * eps;
-    const float inf = __builtin_huge_valf();
-    float ct0 = inf, ct1 = inf, ct2 = inf, ct3 = inf;
-    uint32_t ck0 = 0, ck1 = 0, ck2 = 0, ck3 = 0;
-    float tmin = inf;
-    bool ovf = false;
-    bool occluded = false;
-    float cull = any ? tmax_eps : tmax_eps + win;
+struct Ray {
+    f3 o, d;
+    float tmax, tmax_eps, cull, tmin;
+    float ix, iy, iz, ox, oy, oz, ax, ay, az, px, py, pz;
+    float ct0, ct1, ct2, ct3;
+    uint32_t ck0, ck1, ck2, ck3;
+    bool any, ovf, occluded, nan_ray;
+    uint32_t ref;   // >= 0 (as int): inner node; < -1: leaf; kDone: traversal over
+    int sp;         // stack depth
 
-    const bool nan_ray = (d.x != d.x) || (d.y != d.y) || (d.z != d.z) || (o.x != o.x) || (o.y != o.y) || (o.z != o.z);
-    const BvhNode* __restrict__ nodes = sc.nodes;
-    const TriRecord* __restrict__ tris = sc.tris;
-    const float4* __restrict__ sph = sc.sph;
-    const uint32_t* __restrict__ sph_key = sc.sph_key;
-
-    if (sc.has_bvh && !nan_ray) {
-        const float ix = 1.0f / (fabsf(d.x) > 1e-20f ? d.x : copysignf(1e-20f, d.x));
-        const float iy = 1.0f / (fabsf(d.y) > 1e-20f ? d.y : copysignf(1e-20f, d.y));
-        const float iz = 1.0f / (fabsf(d.z) > 1e-20f ? d.z : copysignf(1e-20f, d.z));
-        const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
-        const float ax = fabsf(ix), ay = fabsf(iy), az = fabsf(iz);
+    __device__ __forceinline__ void start(const DevScene& sc, f3 o_, f3 d_, float tmax_, float eps, bool any_)
+    {
+        const float inf = __builtin_huge_valf();
+        o = o_;
+        d = d_;
+        tmax = tmax_;
+        any = any_;
+        tmax_eps = tmax + eps;
+        cull = any ? tmax_eps : tmax_eps + 4.0f * eps;
+        tmin = inf;
+        ct0 = ct1 = ct2 = ct3 = inf;
+        ck0 = ck1 = ck2 = ck3 = 0;
+        ovf = false;
+        occluded = false;
+        nan_ray = (d.x != d.x) || (d.y != d.y) || (d.z != d.z) || (o.x != o.x) || (o.y != o.y) || (o.z != o.z);
+        ix = 1.0f / (fabsf(d.x) > 1e-20f ? d.x : copysignf(1e-20f, d.x));
+        iy = 1.0f / (fabsf(d.y) > 1e-20f ? d.y : copysignf(1e-20f, d.y));
+        iz = 1.0f / (fabsf(d.z) > 1e-20f ? d.z : copysignf(1e-20f, d.z));
+        ox = o.x * ix;
+        oy = o.y * iy;
+        oz = o.z * iz;
+        ax = fabsf(ix);
+        ay = fabsf(iy);
+        az = fabsf(iz);
         // The sphere test (sphere_t, Assets/Tracer.comp:314-330) rounds its
         // discriminant with an error up to ~60 ulp of |o - c|^2, so from far
-        // away it reports hits up to ~1.4e-3 |o - c| outside the sphere.
-        // The sphere subtree's boxes are widened by 2e-3 (|o - C| + R), which
+        // away it reports hits up to ~1e-3 |o - c| outside the sphere.  The
+        // sphere subtree's boxes are widened by 2e-3 (|o - C| + R), which
         // bounds that distance for every centre c within R of C.
         const f3 oc = sub(o, f3{sc.sph_c[0], sc.sph_c[1], sc.sph_c[2]});
         const float pad = 2e-3f * (sqrtf(dot(oc, oc)) + sc.sph_r);
-        const float px = fminf(pad * ax, 3e38f), py = fminf(pad * ay, 3e38f), pz = fminf(pad * az, 3e38f);
-        uint32_t ref = 0;
-        int sp = 0;
-        while (true) {
-            while (true) {
-                // ref >= 0 (as int): inner node; < -1: leaf; -1 (kDone): finished.
-                const uint32_t nn = (uint32_t)__popcll(__ballot((int)ref >= 0));
-                const uint32_t nl = (uint32_t)__popcll(__ballot((int)ref < -1));
-                if (nn == 0 || (nl > 0 && nl * 8u >= (nl + nn) * kLeafExit8)) break;
-                if ((int)ref >= 0) {
-                    // 32-bit byte offsets (node count < 2^26, checked at
-                    // upload) let the loads use a scalar base address.
-                    const float4* np =
-                        reinterpret_cast<const float4*>(reinterpret_cast<const char*>(nodes) + (ref << 6));
-                    const float4 A = np[0], B = np[1], C = np[2];
-                    const uint4 D = reinterpret_cast<const uint4*>(np)[3];
-                    if (COUNT) {
-                        wc.nodes++;
-                        count_wave(wc.wave_nodes);
-                    }
-                    const float cb = cull + fabsf(cull) * 1e-5f + 1e-5f;
-                    // Slab test on centre/half-extent boxes: t = (c - o) i -+ (e + w) |i|,
-                    // with the widening w only in the sphere subtree (m = 1 there).
-                    const float m = __uint_as_float(D.z);
-                    const float qx = m * px, qy = m * py, qz = m * pz;
-                    const float a0 = __builtin_fmaf(A.x, ix, -ox), a1 = __builtin_fmaf(A.y, ax, qx);
-                    const float a2 = __builtin_fmaf(A.z, iy, -oy), a3 = __builtin_fmaf(A.w, ay, qy);
-                    const float a4 = __builtin_fmaf(B.x, iz, -oz), a5 = __builtin_fmaf(B.y, az, qz);
-                    const float b0 = __builtin_fmaf(B.z, ix, -ox), b1 = __builtin_fmaf(B.w, ax, qx);
-                    const float b2 = __builtin_fmaf(C.x, iy, -oy), b3 = __builtin_fmaf(C.y, ay, qy);
-                    const float b4 = __builtin_fmaf(C.z, iz, -oz), b5 = __builtin_fmaf(C.w, az, qz);
-                    const float n0 = fmaxf(fmaxf(a0 - a1, a2 - a3), fmaxf(a4 - a5, 0.0f));
-                    const float f0 = fminf(fminf(a0 + a1, a2 + a3), fminf(a4 + a5, cb));
-                    const float n1 = fmaxf(fmaxf(b0 - b1, b2 - b3), fmaxf(b4 - b5, 0.0f));
-                    const float f1 = fminf(fminf(b0 + b1, b2 + b3), fminf(b4 + b5, cb));
-                    const bool h0 = n0 <= f0, h1 = n1 <= f1;
-                    // Branch-free step: the far child is always written to
-                    // the free slot above the top (sp < STACK at any inner
-                    // node, since the tree depth bounds the stack), and only
-                    // counted as pushed when both children are hit.
-                    const bool first0 = n0 <= n1;
-                    const uint32_t nearr = (h0 && (!h1 || first0)) ? D.x : D.y;
-                    stk[sp * BLOCK] = first0 ? D.y : D.x;
-                    const uint32_t top = stk[(sp > 0 ? sp - 1 : 0) * BLOCK];
-                    const bool none = !h0 && !h1;
-                    ref = none ? (sp > 0 ? top : kDone) : nearr;
-                    sp = (h0 && h1) ? sp + 1 : (none ? (sp > 0 ? sp - 1 : 0) : sp);
-                }
+        px = fminf(pad * ax, 3e38f);
+        py = fminf(pad * ay, 3e38f);
+        pz = fminf(pad * az, 3e38f);
+        ref = (sc.has_bvh && !nan_ray) ? 0u : kDone;
+        sp = 0;
+    }
+
+    // Visits inner node `ref`: near-first descent with a branch-free step.
+    template <int BLOCK, bool COUNT>
+    __device__ __forceinline__ void node(const BvhNode* __restrict__ nodes, uint32_t* stk, WorkCnt& wc)
+    {
+        // 32-bit byte offsets (node count < 2^26, checked at upload) let the
+        // loads use a scalar base address.
+        const float4* np = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(nodes) + (ref << 6));
+        const float4 A = np[0], B = np[1], C = np[2];
+        const uint4 D = reinterpret_cast<const uint4*>(np)[3];
+        if (COUNT) {
+            wc.nodes++;
+            count_wave(wc.wave_nodes);
+        }
+        const float cb = cull + fabsf(cull) * 1e-5f + 1e-5f;
+        // Slab test on centre/half-extent boxes: t = (c - o) i -+ (e + w) |i|,
+        // with the widening w only in the sphere subtree (m = 1 there).
+        const float m = __uint_as_float(D.z);
+        const float qx = m * px, qy = m * py, qz = m * pz;
+        const float a0 = __builtin_fmaf(A.x, ix, -ox), a1 = __builtin_fmaf(A.y, ax, qx);
+        const float a2 = __builtin_fmaf(A.z, iy, -oy), a3 = __builtin_fmaf(A.w, ay, qy);
+        const float a4 = __builtin_fmaf(B.x, iz, -oz), a5 = __builtin_fmaf(B.y, az, qz);
+        const float b0 = __builtin_fmaf(B.z, ix, -ox), b1 = __builtin_fmaf(B.w, ax, qx);
+        const float b2 = __builtin_fmaf(C.x, iy, -oy), b3 = __builtin_fmaf(C.y, ay, qy);
+        const float b4 = __builtin_fmaf(C.z, iz, -oz), b5 = __builtin_fmaf(C.w, az, qz);
+        const float n0 = fmaxf(fmaxf(a0 - a1, a2 - a3), fmaxf(a4 - a5, 0.0f));
+        const float f0 = fminf(fminf(a0 + a1, a2 + a3), fminf(a4 + a5, cb));
+        const float n1 = fmaxf(fmaxf(b0 - b1, b2 - b3), fmaxf(b4 - b5, 0.0f));
+        const float f1 = fminf(fminf(b0 + b1, b2 + b3), fminf(b4 + b5, cb));
+        const bool h0 = n0 <= f0, h1 = n1 <= f1;
+        // The far child is always written to the free slot above the top
+        // (sp < STACK at any inner node, since the tree depth bounds the
+        // stack), and only counted as pushed when both children are hit.
+        const bool first0 = n0 <= n1;
+        const uint32_t nearr = (h0 && (!h1 || first0)) ? D.x : D.y;
+        stk[sp * BLOCK] = first0 ? D.y : D.x;
+        const uint32_t top = stk[(sp > 0 ? sp - 1 : 0) * BLOCK];
+        const bool none = !h0 && !h1;
+        ref = none ? (sp > 0 ? top : kDone) : nearr;
+        sp = (h0 && h1) ? sp + 1 : (none ? (sp > 0 ? sp - 1 : 0) : sp);
+    }
+
+    // Tests the primitives of leaf `ref`, then pops the stack.
+    template <int BLOCK, bool COUNT>
+    __device__ __forceinline__ void leaf(const DevScene& sc, float eps, uint32_t* stk, WorkCnt& wc)
+    {
+        const float inf = __builtin_huge_valf();
+        const float win = 4.0f * eps;
+        if (COUNT) count_wave(wc.wave_leaves);
+        const uint32_t count = ((ref >> kRefCountShift) & 7u) + 1u;
+        const uint32_t first = ref & kRefFirstMask;
+        for (uint32_t i = 0; i < count; ++i) {
+            float t;
+            uint32_t key;
+            if (ref & kRefSphere) {
+                const float4 s4 = sc.sph[first + i];
+                if (COUNT) wc.spheres++;
+                t = sphere_t(o, d, f3{s4.x, s4.y, s4.z}, s4.w);
+                key = 0x80000000u | sc.sph_key[first + i];
+            } else {
+                const float4* tp = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(sc.tris) +
+                                                                    (first + i) * (uint32_t)sizeof(TriRecord));
+                const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+                if (COUNT) wc.tris++;
+                t = tri_t(o, d, f3{p0.x, p0.y, p0.z}, f3{p1.x, p1.y, p1.z}, f3{p2.x, p2.y, p2.z}, eps);
+                key = __float_as_uint(p0.w);
             }
-            if (__ballot(ref != kDone) == 0) break;
-            if ((int)ref < -1) {
-                // Leaf: test its primitives.
-                if (COUNT) count_wave(wc.wave_leaves);
-                const uint32_t count = ((ref >> kRefCountShift) & 7u) + 1u;
-                const uint32_t first = ref & kRefFirstMask;
-                for (uint32_t i = 0; i < count; ++i) {
-                    float t;
-                    uint32_t key;
-                    if (ref & kRefSphere) {
-                        const float4 s4 = sph[first + i];
-                        if (COUNT) wc.spheres++;
-                        t = sphere_t(o, d, f3{s4.x, s4.y, s4.z}, s4.w);
-                        key = 0x80000000u | sph_key[first + i];
-                    } else {
-                        const float4* tp = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(tris) +
-                                                                            (first + i) * (uint32_t)sizeof(TriRecord));
-                        const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
-                        if (COUNT) wc.tris++;
-                        t = tri_t(o, d, f3{p0.x, p0.y, p0.z}, f3{p1.x, p1.y, p1.z}, f3{p2.x, p2.y, p2.z}, eps);
-                        key = __float_as_uint(p0.w);
-                    }
-                    if (!(t > eps)) continue;
-                    if (any) {
-                        if (t < tmax_eps) occluded = true;
-                    } else if (t < fminf(tmin, tmax_eps) + win) {
-                        if (t < tmin) {
-                            tmin = t;
-                            const float lim = tmin + win;
-                            ct0 = ct0 < lim ? ct0 : inf;
-                            ct1 = ct1 < lim ? ct1 : inf;
-                            ct2 = ct2 < lim ? ct2 : inf;
-                            ct3 = ct3 < lim ? ct3 : inf;
-                        }
-                        if (ct0 == inf) { ct0 = t; ck0 = key; }
-                        else if (ct1 == inf) { ct1 = t; ck1 = key; }
-                        else if (ct2 == inf) { ct2 = t; ck2 = key; }
-                        else if (ct3 == inf) { ct3 = t; ck3 = key; }
-                        else {
-                            // List full: keep the four smallest t (documented near-tie).
-                            ovf = true;
-                            float mx = ct0; int mi = 0;
-                            if (ct1 > mx) { mx = ct1; mi = 1; }
-                            if (ct2 > mx) { mx = ct2; mi = 2; }
-                            if (ct3 > mx) { mx = ct3; mi = 3; }
-                            if (t < mx) {
-                                if (mi == 0) { ct0 = t; ck0 = key; }
-                                else if (mi == 1) { ct1 = t; ck1 = key; }
-                                else if (mi == 2) { ct2 = t; ck2 = key; }
-                                else { ct3 = t; ck3 = key; }
-                            }
-                        }
-                        cull = fminf(tmin, tmax_eps) + win;
+            if (!(t > eps)) continue;
+            if (any) {
+                if (t < tmax_eps) occluded = true;
+            } else if (t < fminf(tmin, tmax_eps) + win) {
+                if (t < tmin) {
+                    tmin = t;
+                    const float lim = tmin + win;
+                    ct0 = ct0 < lim ? ct0 : inf;
+                    ct1 = ct1 < lim ? ct1 : inf;
+                    ct2 = ct2 < lim ? ct2 : inf;
+                    ct3 = ct3 < lim ? ct3 : inf;
+                }
+                if (ct0 == inf) { ct0 = t; ck0 = key; }
+                else if (ct1 == inf) { ct1 = t; ck1 = key; }
+                else if (ct2 == inf) { ct2 = t; ck2 = key; }
+                else if (ct3 == inf) { ct3 = t; ck3 = key; }
+                else {
+                    // List full: keep the four smallest t (documented near-tie).
+                    ovf = true;
+                    float mx = ct0; int mi = 0;
+                    if (ct1 > mx) { mx = ct1; mi = 1; }
+                    if (ct2 > mx) { mx = ct2; mi = 2; }
+                    if (ct3 > mx) { mx = ct3; mi = 3; }
+                    if (t < mx) {
+                        if (mi == 0) { ct0 = t; ck0 = key; }
+                        else if (mi == 1) { ct1 = t; ck1 = key; }
+                        else if (mi == 2) { ct2 = t; ck2 = key; }
+                        else { ct3 = t; ck3 = key; }
                     }
                 }
-                if (occluded) {
-                    ref = kDone;
-                } else {
-                    ref = sp > 0 ? stk[(sp - 1) * BLOCK] : kDone;
-                    sp = sp > 0 ? sp - 1 : 0;
-                }
+                cull = fminf(tmin, tmax_eps) + win;
             }
+        }
+        if (occluded) {
+            ref = kDone;
+        } else {
+            ref = sp > 0 ? stk[(sp - 1) * BLOCK] : kDone;
+            sp = sp > 0 ? sp - 1 : 0;
         }
     }
 
-    TravOut r;
-    const uint32_t np = sc.n_planes;
-    const vkr_plane* __restrict__ planes = sc.planes;
-    if (any) {
-        if (!occluded && !nan_ray) {
-            const float lim = tmax - eps;
-            for (uint32_t i = 0; i < np; ++i) {
-                if (COUNT) wc.planes++;
-                const float t = plane_t(o, d, ld3(planes[i].N), planes[i].len);
-                if ((t > eps) && (t < lim)) { occluded = true; break; }
+    // After traversal: the planes (a linear list) and, for closest-hit rays,
+    // the replay of the candidates in scan order.
+    template <bool COUNT>
+    __device__ __forceinline__ TravOut finish(const DevScene& sc, float eps, WorkCnt& wc)
+    {
+        TravOut r;
+        const uint32_t np = sc.n_planes;
+        const vkr_plane* __restrict__ planes = sc.planes;
+        if (any) {
+            if (!occluded && !nan_ray) {
+                const float lim = tmax - eps;
+                for (uint32_t i = 0; i < np; ++i) {
+                    if (COUNT) wc.planes++;
+                    const float t = plane_t(o, d, ld3(planes[i].N), planes[i].len);
+                    if ((t > eps) && (t < lim)) { occluded = true; break; }
+                }
             }
+            r.hit = occluded;
+            r.prim = kMiss;
+            r.t = tmax;
+            return r;
         }
-        r.hit = occluded;
-        r.prim = kMiss;
-        r.t = tmax;
-        return r;
-    }
-    // Replay the scan's sequential rule over the candidates in scan order.
-    cswap(ct0, ck0, ct1, ck1);
-    cswap(ct2, ck2, ct3, ck3);
-    cswap(ct0, ck0, ct2, ck2);
-    cswap(ct1, ck1, ct3, ck3);
-    cswap(ct1, ck1, ct2, ck2);
-    float best = tmax;
-    uint32_t win_key = kMiss;
-    const float top = tmin + win - eps;
+        cswap(ct0, ck0, ct1, ck1);
+        cswap(ct2, ck2, ct3, ck3);
+        cswap(ct0, ck0, ct2, ck2);
+        cswap(ct1, ck1, ct3, ck3);
+        cswap(ct1, ck1, ct2, ck2);
+        float best = tmax;
+        uint32_t win_key = kMiss;
+        const float top = tmin + 4.0f * eps - eps;
 #define VKR_REPLAY(T, K)                          \
     if ((T) < best + eps) {                       \
         best = (T);                               \
         win_key = (K);                            \
         if (best >= top) ovf = true;              \
     }
-    VKR_REPLAY(ct0, ck0)
-    VKR_REPLAY(ct1, ck1)
-    VKR_REPLAY(ct2, ck2)
-    VKR_REPLAY(ct3, ck3)
+        VKR_REPLAY(ct0, ck0)
+        VKR_REPLAY(ct1, ck1)
+        VKR_REPLAY(ct2, ck2)
+        VKR_REPLAY(ct3, ck3)
 #undef VKR_REPLAY
-    uint32_t prim = kMiss;
-    if (win_key != kMiss) prim = (win_key & 0x80000000u) ? ((1u << 30) | (win_key & 0x7FFFFFFFu)) : win_key;
-    if (!nan_ray) {
-        for (uint32_t i = 0; i < np; ++i) {
-            if (COUNT) wc.planes++;
-            const float t = plane_t(o, d, ld3(planes[i].N), planes[i].len);
-            if ((t > eps) && (t < best - eps)) {
-                best = t;
-                prim = (2u << 30) | i;
+        uint32_t prim = kMiss;
+        if (win_key != kMiss) prim = (win_key & 0x80000000u) ? ((1u << 30) | (win_key & 0x7FFFFFFFu)) : win_key;
+        if (!nan_ray) {
+            for (uint32_t i = 0; i < np; ++i) {
+                if (COUNT) wc.planes++;
+                const float t = plane_t(o, d, ld3(planes[i].N), planes[i].len);
+                if ((t > eps) && (t < best - eps)) {
+                    best = t;
+                    prim = (2u << 30) | i;
+                }
             }
         }
+        if (ovf) wc.overflow++;
+        r.hit = prim != kMiss;
+        r.prim = prim;
+        r.t = best;
+        return r;
     }
-    if (ovf) wc.overflow++;
-    r.hit = prim != kMiss;
-    r.prim = prim;
-    r.t = best;
-    return r;
+};
+
+// The node loop's exit test: a wave leaves it when no lane is at an inner
+// node, or when at least kLeafExit8/8 of the lanes that still traverse wait
+// (at a leaf, or for a new ray).
+__device__ __forceinline__ bool leave_nodes(uint32_t nn, uint32_t nw)
+{
+    return nn == 0 || (nw > 0 && nw * 8u >= (nw + nn) * kLeafExit8);
+}
+
+// One ray per lane (the adjoint kernel's replay).
+template <int BLOCK, bool COUNT>
+__device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, float tmax, float eps, bool any,
+                                            uint32_t* stk, WorkCnt& wc)
+{
+    Ray r;
+    r.start(sc, o, d, tmax, eps, any);
+    while (true) {
+        while (true) {
+            const uint32_t nn = (uint32_t)__popcll(__ballot((int)r.ref >= 0));
+            const uint32_t nl = (uint32_t)__popcll(__ballot((int)r.ref < -1));
+            if (leave_nodes(nn, nl)) break;
+            if ((int)r.ref >= 0) r.node<BLOCK, COUNT>(sc.nodes, stk, wc);
+        }
+        if (__ballot(r.ref != kDone) == 0) break;
+        if ((int)r.ref < -1) r.leaf<BLOCK, COUNT>(sc, eps, stk, wc);
+    }
+    return r.finish<COUNT>(sc, eps, wc);
+}
+
+constexpr uint32_t kNone = 0xFFFFu;
+
+// Traces every ray the lanes of a wave queued for this step.  Lanes take rays
+// from the wave's pool (pool[0..total), entries (owner lane << 4) | slot)
+// until it is empty, so a step costs the wave's total work plus the last
+// rays' tail instead of its longest lane.  A shadow ray sets bit `slot` of
+// occ[owner] when occluded; the closest-hit ray of lane `owner` writes
+// (prim, t) to res[owner].  `wrec` is the record of the wave's lane 0.
+template <int BLOCK, bool COUNT>
+__device__ __forceinline__ void traverse_pool(const DevScene& sc, float eps, uint32_t total, const uint16_t* pool,
+                                              uint32_t* occ, uint2* res, const char* __restrict__ wrec,
+                                              uint32_t* stk, WorkCnt& wc)
+{
+    Ray r;
+    r.ref = kDone;
+    uint32_t cur = kNone;
+    uint32_t next = 0;   // wave-uniform pool cursor
+    while (true) {
+        // Post finished rays and hand out new ones until every lane is busy
+        // or the pool is empty.
+        while (true) {
+            if (cur != kNone && r.ref == kDone) {
+                const TravOut t = r.finish<COUNT>(sc, eps, wc);
+                const uint32_t owner = cur >> 4;
+                if (r.any) {
+                    if (t.hit) atomicOr(occ + owner, 1u << (cur & 15u));
+                } else {
+                    res[owner] = make_uint2(t.prim, __float_as_uint(t.t));
+                }
+                cur = kNone;
+            }
+            const unsigned long long want = __ballot(cur == kNone);
+            if (want != 0 && next < total) {
+                const uint32_t rank = (uint32_t)__popcll(want & ((1ull << __lane_id()) - 1ull));
+                if (cur == kNone && next + rank < total) {
+                    cur = pool[next + rank];
+                    const char* rec = wrec + (cur >> 4) * kLaneBytes;
+                    const float4 h = *reinterpret_cast<const float4*>(rec + kRecHead);
+                    const float4 s = *reinterpret_cast<const float4*>(rec + kRecSlots + (cur & 15u) * 32u);
+                    r.start(sc, f3{h.x, h.y, h.z}, f3{s.x, s.y, s.z}, s.w, eps, h.w != 0.0f);
+                }
+                next += (uint32_t)__popcll(want);
+            }
+            if (__ballot(cur != kNone && r.ref == kDone) == 0) break;
+        }
+        if (__ballot(cur != kNone) == 0) break;
+        const bool more = next < total;
+        while (true) {
+            const uint32_t nn = (uint32_t)__popcll(__ballot((int)r.ref >= 0));
+            const uint32_t nw = (uint32_t)__popcll(__ballot((int)r.ref < -1 || (more && r.ref == kDone)));
+            if (leave_nodes(nn, nw)) break;
+            if ((int)r.ref >= 0) r.node<BLOCK, COUNT>(sc.nodes, stk, wc);
+        }
+        if ((int)r.ref < -1) r.leaf<BLOCK, COUNT>(sc, eps, stk, wc);
+    }
 }
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
