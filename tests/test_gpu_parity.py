@@ -279,6 +279,17 @@ def test_schedules_identical(ctx, torch_cuda):
                     assert np.array_equal(g[k].view(np.uint8), base[k].view(np.uint8)), (sched, block, chunk, k)
 
 
+def test_plain_launch_split(ctx, torch_cuda):
+    """One pixel per lane above 2^21 pixels runs as several launches (bounded
+    ray records); the image is identical to the persistent schedule's."""
+    wl = scenes.reference_scene(2048, 1088, spp=1, max_depth=2)
+    sid = ctx.upload(wl.scene)
+    a = gpu_trace(ctx, sid, wl, sched=vkr.SCHED_PLAIN)
+    b = gpu_trace(ctx, sid, wl, sched=vkr.SCHED_PERSISTENT)
+    for k in ("rgba8", "radiance", "prim_id", "hit_t", "ray_count"):
+        assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), k
+
+
 def test_odd_sizes_and_x0_column(ctx, torch_cuda):
     for (w, h) in ((1, 1), (7, 5), (33, 17)):
         wl = scenes.reference_scene(w, h, spp=3, max_depth=4)

@@ -62,6 +62,10 @@ struct DevCounters {
 // 1 refills too often).
 constexpr uint32_t kDefaultChunk = 8u;
 
+// One-pixel-per-lane launches cover at most this many lanes each, which
+// bounds their ray records (2^21 x kLaneBytes = 2.4 GB).
+constexpr uint32_t kMaxPlainLanes = 1u << 21;
+
 // Per-lane ray record of the path kernel (one per lane of a launch, in
 // global memory):
 //   [0, 64)    path state parked while the wave traces
@@ -69,7 +73,7 @@ constexpr uint32_t kDefaultChunk = 8u;
 //              closest-hit ray
 //   [80, ...)  kRaySlots slots of 32 bytes: (d xyz, t_max), (next-event
 //              contribution xyz, 0)
-constexpr uint32_t kRaySlots = 8;
+constexpr uint32_t kRaySlots = 32;
 constexpr uint32_t kRecHead = 64u;
 constexpr uint32_t kRecSlots = 80u;
 constexpr uint32_t kLaneBytes = kRecSlots + 32u * kRaySlots;
@@ -81,6 +85,7 @@ struct PathLaunch {
     uint32_t n_jobs;
     uint32_t total;               // pool indices (64 per 8x8 tile)
     uint32_t persistent;          // 0: one pixel per lane; 1: refill from pool
+    uint32_t base;                // one pixel per lane: pool index of the launch's lane 0
     uint32_t chunk;               // refill when at least this many lanes idle
     uint32_t* pool;               // persistent pool cursor (device)
     DevCounters* counters;

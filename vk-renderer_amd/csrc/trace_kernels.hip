@@ -38,7 +38,7 @@ template <int BLOCK, int STACK, bool COUNT, bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
 {
     __shared__ uint32_t s_stack[STACK * BLOCK];
-    __shared__ uint16_t s_pool[BLOCK * kRaySlots];   // per wave: the step's rays, (lane << 4) | slot
+    __shared__ uint32_t s_pre[BLOCK];                // per lane: index of its first queued ray in the wave
     __shared__ uint32_t s_occ[BLOCK];                // per lane: occluded shadow rays (bit per slot)
     __shared__ uint2 s_res[BLOCK];                   // per lane: closest hit (prim, t)
     uint32_t* stk = s_stack + threadIdx.x;
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
 
     if (!A.persistent) {
         const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
-        const uint32_t g = b * BLOCK + threadIdx.x;
+        const uint32_t g = A.base + b * BLOCK + threadIdx.x;
         if (g < A.total) claim(g);
     }
 
@@ -312,14 +312,13 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
         {
             const unsigned long long lt = (1ull << __lane_id()) - 1ull;
             uint32_t pre = 0, tot = 0;
-            for (uint32_t b = 0; b < 4u; ++b) {   // nray <= kRaySlots = 8
+            for (uint32_t b = 0; (1u << b) <= kRaySlots; ++b) {   // the bits of nray (<= kRaySlots)
                 const unsigned long long m = __ballot((nray >> b) & 1u);
                 pre += (uint32_t)__popcll(m & lt) << b;
                 tot += (uint32_t)__popcll(m) << b;
             }
             s_occ[threadIdx.x] = 0u;
-            for (uint32_t k = 0; k < nray; ++k)
-                s_pool[wbase * kRaySlots + pre + k] = (uint16_t)((__lane_id() << 4) | k);
+            s_pre[threadIdx.x] = pre;
             // Path state that tracing does not need waits in the record, so it
             // holds no registers meanwhile.
             float4* park = reinterpret_cast<float4*>(rec);
@@ -332,8 +331,7 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            traverse_pool<BLOCK, COUNT>(j.sc, j.eps, tot, s_pool + wbase * kRaySlots, s_occ + wbase, s_res + wbase,
-                                        wrec, stk, wc);
+            traverse_pool<BLOCK, COUNT>(j.sc, j.eps, tot, s_pre + wbase, s_occ + wbase, s_res + wbase, wrec, stk, wc);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

@@ -309,13 +309,13 @@ __device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, floa
 constexpr uint32_t kNone = 0xFFFFu;
 
 // Traces every ray the lanes of a wave queued for this step.  Lanes take rays
-// from the wave's pool (pool[0..total), entries (owner lane << 4) | slot)
-// until it is empty, so a step costs the wave's total work plus the last
+// 0..total of the wave (lane l queued rays pre[l] .. pre[l+1]-1) until none
+// is left, so a step costs the wave's total work plus the last
 // rays' tail instead of its longest lane.  A shadow ray sets bit `slot` of
 // occ[owner] when occluded; the closest-hit ray of lane `owner` writes
 // (prim, t) to res[owner].  `wrec` is the record of the wave's lane 0.
 template <int BLOCK, bool COUNT>
-__device__ __forceinline__ void traverse_pool(const DevScene& sc, float eps, uint32_t total, const uint16_t* pool,
+__device__ __forceinline__ void traverse_pool(const DevScene& sc, float eps, uint32_t total, const uint32_t* pre,
                                               uint32_t* occ, uint2* res, const char* __restrict__ wrec,
                                               uint32_t* stk, WorkCnt& wc)
 {
@@ -329,9 +329,9 @@ __device__ __forceinline__ void traverse_pool(const DevScene& sc, float eps, uin
         while (true) {
             if (cur != kNone && r.ref == kDone) {
                 const TravOut t = r.finish<COUNT>(sc, eps, wc);
-                const uint32_t owner = cur >> 4;
+                const uint32_t owner = cur >> 5;
                 if (r.any) {
-                    if (t.hit) atomicOr(occ + owner, 1u << (cur & 15u));
+                    if (t.hit) atomicOr(occ + owner, 1u << (cur & 31u));
                 } else {
                     res[owner] = make_uint2(t.prim, __float_as_uint(t.t));
                 }
@@ -341,10 +341,17 @@ __device__ __forceinline__ void traverse_pool(const DevScene& sc, float eps, uin
             if (want != 0 && next < total) {
                 const uint32_t rank = (uint32_t)__popcll(want & ((1ull << __lane_id()) - 1ull));
                 if (cur == kNone && next + rank < total) {
-                    cur = pool[next + rank];
-                    const char* rec = wrec + (cur >> 4) * kLaneBytes;
+                    // Ray e belongs to the last lane whose first queued ray is <= e.
+                    const uint32_t e = next + rank;
+                    uint32_t lo = 0, plo = 0;
+                    for (uint32_t step = 32; step > 0; step >>= 1) {
+                        const uint32_t p = pre[lo + step];
+                        if (p <= e) { lo += step; plo = p; }
+                    }
+                    cur = (lo << 5) | (e - plo);
+                    const char* rec = wrec + (cur >> 5) * kLaneBytes;
                     const float4 h = *reinterpret_cast<const float4*>(rec + kRecHead);
-                    const float4 s = *reinterpret_cast<const float4*>(rec + kRecSlots + (cur & 15u) * 32u);
+                    const float4 s = *reinterpret_cast<const float4*>(rec + kRecSlots + (cur & 31u) * 32u);
                     r.start(sc, f3{h.x, h.y, h.z}, f3{s.x, s.y, s.z}, s.w, eps, h.w != 0.0f);
                 }
                 next += (uint32_t)__popcll(want);
