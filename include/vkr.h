@@ -216,7 +216,12 @@ enum {
     /* RR / dielectric decisions are read from outputs.decisions instead of
      * being drawn (the random numbers are still drawn, so the stream is
      * unchanged).  Used for finite differences with frozen branches. */
-    VKR_TRACE_REPLAY_DECISIONS = 1u << 1
+    VKR_TRACE_REPLAY_DECISIONS = 1u << 1,
+    /* Shadow rays: with at least 2 emitters a wave traces up to 32 rays per
+     * lane and step from a shared pool, else one ray per lane and step.
+     * These force one or the other; the results are identical either way. */
+    VKR_TRACE_RAY_POOL         = 1u << 2,
+    VKR_TRACE_NO_RAY_POOL      = 1u << 3
 };
 
 typedef struct vkr_trace_params {

@@ -279,6 +279,21 @@ def test_schedules_identical(ctx, torch_cuda):
                     assert np.array_equal(g[k].view(np.uint8), base[k].view(np.uint8)), (sched, block, chunk, k)
 
 
+def test_ray_pool_variants_identical(ctx, torch_cuda):
+    """The pooled kernel (several shadow rays per lane and step, traced from a
+    wave-shared pool) and the single-ray kernel give identical bits, on a
+    scene with 18 emitters and on one with a single emitter."""
+    for wl in (scenes.spheres64(), scenes.reference_scene(256, 192, spp=2, max_depth=4)):
+        sid = ctx.upload(wl.scene)
+        a = gpu_trace(ctx, sid, wl, flags=vkr.TRACE_RAY_POOL)
+        b = gpu_trace(ctx, sid, wl, flags=vkr.TRACE_NO_RAY_POOL)
+        for k in ("rgba8", "radiance", "prim_id", "hit_t", "ray_count"):
+            assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), (wl.name, k)
+    r = oracle_ffi.trace(wl.scene, wl.frame, wl.width, wl.height, wl.spp, wl.max_depth)
+    st = compare(a, r)
+    assert st["prim_id"] == 1.0 and st["radiance_bits"] == 1.0, st
+
+
 def test_plain_launch_split(ctx, torch_cuda):
     """One pixel per lane above 2^21 pixels runs as several launches (bounded
     ray records); the image is identical to the persistent schedule's."""

@@ -251,7 +251,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(const AdjointLaunch A)
         }
 
         TravOut tr{kMiss, 0.0f, false};
-        if (has && need) tr = traverse<BLOCK, false>(j.sc, o, d, tmax, j.eps, any, stk, wc);
+        if (has && need) tr = traverse<BLOCK, false>(j.sc, o, d, tmax, j.eps, any, true, stk, wc);
         if (!has) continue;
 
         bool fin_diffuse = false, do_rr = false, path_end = false;

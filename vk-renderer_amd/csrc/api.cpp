@@ -292,21 +292,23 @@ vkr_error vkr_trace_frame(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd
         const bool persistent = sched != VKR_SCHED_PLAIN;
         a.persistent = persistent ? 1u : 0u;
         a.chunk = chunk ? chunk : kDefaultChunk;
-        const bool count = (ctx->flags & VKR_CTX_COUNT_WORK) != 0;
+        const PathVariant v{block, stack, (ctx->flags & VKR_CTX_COUNT_WORK) != 0, false,
+                            use_ray_pool(params->flags, s->dev.n_emitters)};
         // Persistent: one resident grid.  One pixel per lane: launches of at
         // most kMaxPlainLanes lanes each, in pool order.
         const uint32_t step = persistent ? a.total : kMaxPlainLanes;
-        const uint32_t grid = persistent ? std::min(persistent_grid(block, stack, count, false, ctx->device),
-                                                    (a.total + block - 1u) / block)
+        const uint32_t grid = persistent ? std::min(persistent_grid(v, ctx->device), (a.total + block - 1u) / block)
                                          : (std::min(a.total, step) + block - 1u) / block;
-        const vkr_error le = ensure_lanes(ctx, (size_t)grid * block);
-        if (le != VKR_SUCCESS) return le;
-        a.lanes = ctx->lanes.as<char>();
+        if (v.pool) {
+            const vkr_error le = ensure_lanes(ctx, (size_t)grid * block);
+            if (le != VKR_SUCCESS) return le;
+            a.lanes = ctx->lanes.as<char>();
+        }
         VKR_HIP(hipEventRecord(ctx->ev0, st));
         for (uint32_t base = 0; base < a.total; base += step) {
             a.base = base;
             const uint32_t g = persistent ? grid : (std::min(a.total - base, step) + block - 1u) / block;
-            VKR_HIP(launch_path(a, block, stack, count, false, g, st));
+            VKR_HIP(launch_path(a, v, g, st));
         }
         VKR_HIP(hipEventRecord(ctx->ev1, st));
     }

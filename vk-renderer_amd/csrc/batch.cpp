@@ -25,6 +25,7 @@ struct BatchPlan {
     std::vector<uint32_t> start;  // n + 1 pool offsets
     uint32_t stack = 32;
     uint64_t total = 0;
+    bool pool = false;            // some job wants the pooled kernel
 };
 
 vkr_error plan_batch(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n, BatchPlan& plan)
@@ -45,6 +46,7 @@ vkr_error plan_batch(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n, BatchP
         if (!s) return fail(VKR_INVALID_ARGUMENT, "unknown scene in batch job");
         plan.jobs[i] = make_job(*s, jb.frame_data, jb.params, &jb.outputs);
         plan.stack = std::max(plan.stack, s->dev.stack_needed);
+        plan.pool = plan.pool || use_ray_pool(jb.params.flags, s->dev.n_emitters);
         plan.start[i] = (uint32_t)off;
         off += (uint64_t)plan.jobs[i].n_tiles * 64u;
         if (off >= 0xFFFFFFF0ull) return fail(VKR_INVALID_ARGUMENT, "batch exceeds 2^32 pool entries");
@@ -69,10 +71,15 @@ PathLaunch batch_launch(const BatchPlan& plan, const JobDesc* djobs, const uint3
     return a;
 }
 
-uint32_t batch_grid(const BatchPlan& plan, uint32_t block, bool count, int device)
+PathVariant batch_variant(const BatchPlan& plan, uint32_t block, bool count)
 {
-    const uint32_t need = (uint32_t)((plan.total + block - 1u) / block);
-    return std::max(1u, std::min(persistent_grid(block, plan.stack, count, true, device), need));
+    return PathVariant{block, plan.stack, count, true, plan.pool};
+}
+
+uint32_t batch_grid(const BatchPlan& plan, const PathVariant& v, int device)
+{
+    const uint32_t need = (uint32_t)((plan.total + v.block - 1u) / v.block);
+    return std::max(1u, std::min(persistent_grid(v, device), need));
 }
 
 bool valid_block(uint32_t b) { return b == 0 || b == 64 || b == 128 || b == 256; }
@@ -121,13 +128,16 @@ vkr_error vkr_trace_batch(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n_jo
     const bool count = (ctx->flags & VKR_CTX_COUNT_WORK) != 0;
     PathLaunch a = batch_launch(plan, ctx->jobs.as<JobDesc>(), ctx->job_start.as<uint32_t>(), n_jobs, chunk, pool,
                                 cnt);
-    const uint32_t grid = batch_grid(plan, blk, count, ctx->device);
-    const vkr_error le = ensure_lanes(ctx, (size_t)grid * blk);
-    if (le != VKR_SUCCESS) return le;
-    a.lanes = ctx->lanes.as<char>();
+    const PathVariant v = batch_variant(plan, blk, count);
+    const uint32_t grid = batch_grid(plan, v, ctx->device);
+    if (v.pool) {
+        const vkr_error le = ensure_lanes(ctx, (size_t)grid * blk);
+        if (le != VKR_SUCCESS) return le;
+        a.lanes = ctx->lanes.as<char>();
+    }
     VKR_HIP(launch_zero_counters(cnt, pool, st));
     VKR_HIP(hipEventRecord(ctx->ev0, st));
-    VKR_HIP(launch_path(a, blk, plan.stack, count, true, grid, st));
+    VKR_HIP(launch_path(a, v, grid, st));
     VKR_HIP(hipEventRecord(ctx->ev1, st));
     ctx->timed = true;
     ctx->last_counters = nullptr;
@@ -158,13 +168,16 @@ vkr_error vkr_batch_graph_create(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32
     const bool count = (ctx->flags & VKR_CTX_COUNT_WORK) != 0;
     PathLaunch a = batch_launch(plan, g->jobs.as<JobDesc>(), g->job_start.as<uint32_t>(), n_jobs, chunk,
                                 g->pool.as<uint32_t>(), g->counters.as<DevCounters>());
-    const uint32_t grid = batch_grid(plan, blk, count, ctx->device);
-    VKR_HIP(g->lanes.alloc((size_t)grid * blk * kLaneBytes));   // the graph's own records
-    a.lanes = g->lanes.as<char>();
+    const PathVariant v = batch_variant(plan, blk, count);
+    const uint32_t grid = batch_grid(plan, v, ctx->device);
+    if (v.pool) {
+        VKR_HIP(g->lanes.alloc((size_t)grid * blk * kLaneBytes));   // the graph's own records
+        a.lanes = g->lanes.as<char>();
+    }
     hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
         const hipError_t e1 = launch_zero_counters(g->counters.as<DevCounters>(), g->pool.as<uint32_t>(), cap);
-        const hipError_t e2 = launch_path(a, blk, plan.stack, count, true, grid, cap);
+        const hipError_t e2 = launch_path(a, v, grid, cap);
         e = hipStreamEndCapture(cap, &g->graph);
         if (e == hipSuccess && e1 != hipSuccess) e = e1;
         if (e == hipSuccess && e2 != hipSuccess) e = e2;

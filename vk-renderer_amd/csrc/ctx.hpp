@@ -169,6 +169,8 @@ inline vkr_error validate_params(const vkr_trace_params* p)
     if (p->chunk > 64) return fail(VKR_INVALID_ARGUMENT, "chunk must be 0..64");
     if ((p->flags & VKR_TRACE_RECORD_DECISIONS) && (p->flags & VKR_TRACE_REPLAY_DECISIONS))
         return fail(VKR_INVALID_ARGUMENT, "record and replay decisions are exclusive");
+    if ((p->flags & VKR_TRACE_RAY_POOL) && (p->flags & VKR_TRACE_NO_RAY_POOL))
+        return fail(VKR_INVALID_ARGUMENT, "VKR_TRACE_RAY_POOL and VKR_TRACE_NO_RAY_POOL are exclusive");
     return VKR_SUCCESS;
 }
 
@@ -223,6 +225,16 @@ inline vkr_error ensure_lanes(vkr_ctx* ctx, size_t lanes)
     VKR_HIP(hipDeviceSynchronize());
     VKR_HIP(ctx->lanes.alloc(bytes));
     return VKR_SUCCESS;
+}
+
+// Whether a launch uses the pooled variant of the path kernel: forced by the
+// trace flags, else when the scene has at least 2 emitters (with one, no lane
+// ever queues more than one ray and the pool is pure bookkeeping).
+inline bool use_ray_pool(uint32_t flags, uint32_t n_emitters)
+{
+    if (flags & VKR_TRACE_RAY_POOL) return true;
+    if (flags & VKR_TRACE_NO_RAY_POOL) return false;
+    return n_emitters >= 2u;
 }
 
 inline const TuneEntry* find_tuned(const vkr_ctx* ctx, uint32_t scene, const vkr_trace_params& p)

@@ -103,8 +103,15 @@ struct AdjointLaunch {
     uint32_t total;               // pool indices (64 per 8x8 tile)
 };
 
-hipError_t launch_path(const PathLaunch& a, uint32_t block, uint32_t stack, bool count, bool batch,
-                       uint32_t grid, hipStream_t s);
+// One instantiation of the path kernel.  pool: a wave traces up to
+// kRaySlots rays per lane and step from a shared pool (pays off with several
+// emitters); else one ray per lane and step.  Results are identical.
+struct PathVariant {
+    uint32_t block, stack;
+    bool count, batch, pool;
+};
+
+hipError_t launch_path(const PathLaunch& a, const PathVariant& v, uint32_t grid, hipStream_t s);
 hipError_t launch_whitted(const JobDesc& j, DevCounters* c, hipStream_t s);
 hipError_t launch_filter(const uint8_t* cur, const uint8_t* prev, uint32_t sw, uint32_t sh, uint8_t* out,
                          uint32_t dw, uint32_t dh, hipStream_t s);
@@ -112,7 +119,7 @@ hipError_t launch_adjoint(const AdjointLaunch& a, uint32_t stack, hipStream_t s)
 hipError_t launch_debug(uint32_t op, const float* in, float* out, uint32_t n, hipStream_t s);
 hipError_t launch_zero_counters(DevCounters* c, uint32_t* pool, hipStream_t s);
 // Occupancy-derived persistent grid size for (block, stack).
-uint32_t persistent_grid(uint32_t block, uint32_t stack, bool count, bool batch, int device);
+uint32_t persistent_grid(const PathVariant& v, int device);
 // Largest stack depth with a compiled kernel variant (0 when none fits).
 uint32_t stack_variant(uint32_t needed);
 

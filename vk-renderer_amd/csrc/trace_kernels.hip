@@ -34,8 +34,8 @@ namespace vkr {
 // wave's rays together, (4) shade, then run the shared tail blocks (finish
 // diffuse -> Russian roulette -> end of path -> end of pixel) at most once
 // each.
-template <int BLOCK, int STACK, bool COUNT, bool BATCH>
-__global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
+template <int BLOCK, int STACK, bool COUNT, bool BATCH, bool POOL>
+__device__ __forceinline__ void path_program(const PathLaunch& A)
 {
     __shared__ uint32_t s_stack[STACK * BLOCK];
     __shared__ uint32_t s_pre[BLOCK];                // per lane: index of its first queued ray in the wave
@@ -148,14 +148,25 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
                 drep = j.decisions[((size_t)py * j.W + px) * j.spp + smp];
         }
 
-        uint32_t nray = 0;   // rays this lane queues for the step
+        // This step's rays: with POOL up to kRaySlots in the lane's record,
+        // else at most one, kept in registers (o0, d0, t0max; c0 = its
+        // next-event contribution).
+        uint32_t nray = 0;
+        f3 o0{}, d0{}, c0{};
+        float t0max = 0.0f;
         if (has) {
             if (phase == kPhTrace) {
                 acc = clamps(acc, 0.0f, 1.0f);                      // :441
                 const float dp1 = (float)(depth + 1u);
                 const float tmax = 3000.0f / (dp1 * dp1);            // :444
-                *reinterpret_cast<float4*>(rec + kRecHead) = make_float4(ro.x, ro.y, ro.z, 0.0f);
-                *reinterpret_cast<float4*>(rec + kRecSlots) = make_float4(rd.x, rd.y, rd.z, tmax);
+                if constexpr (POOL) {
+                    *reinterpret_cast<float4*>(rec + kRecHead) = make_float4(ro.x, ro.y, ro.z, 0.0f);
+                    *reinterpret_cast<float4*>(rec + kRecSlots) = make_float4(rd.x, rd.y, rd.z, tmax);
+                } else {
+                    o0 = ro;
+                    d0 = rd;
+                    t0max = tmax;
+                }
                 nray = 1;
                 ++rays_px;
                 ++c_rays;
@@ -166,9 +177,10 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
                 // A contribution of exactly +-0 cannot change e whatever the
                 // ray's outcome, so that ray is counted but not traversed.
                 // Rays that must be traced are queued with their
-                // contributions, up to kRaySlots per step, and added to e in
-                // emitter order once their outcomes are known (the same
-                // additions in the same order as the reference's loop).
+                // contributions (up to kRaySlots per step with POOL, else
+                // one) and added to e in emitter order once their outcomes
+                // are known: the same additions in the same order as the
+                // reference's loop.
                 const f3 alb = ld3(hm->albedo);
                 const float rough = hm->roughness, metal = hm->metalness;
                 const f3 V = normalize(sub(ld3(j.cam_pos), P));
@@ -298,18 +310,41 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
                     ++ei;
                     if (!(cnee.x == 0.0f && cnee.y == 0.0f && cnee.z == 0.0f)) {
                         ++c_traced;
-                        float4* sl = reinterpret_cast<float4*>(rec + kRecSlots + nray * 32u);
-                        sl[0] = make_float4(L.x, L.y, L.z, tl);
-                        sl[1] = make_float4(cnee.x, cnee.y, cnee.z, 0.0f);
-                        if (++nray >= kRaySlots) break;
+                        if constexpr (POOL) {
+                            float4* sl = reinterpret_cast<float4*>(rec + kRecSlots + nray * 32u);
+                            sl[0] = make_float4(L.x, L.y, L.z, tl);
+                            sl[1] = make_float4(cnee.x, cnee.y, cnee.z, 0.0f);
+                            if (++nray >= kRaySlots) break;
+                        } else {
+                            o0 = P;
+                            d0 = L;
+                            t0max = tl;
+                            c0 = cnee;
+                            nray = 1;
+                            break;
+                        }
                     }
                 }
-                *reinterpret_cast<float4*>(rec + kRecHead) = make_float4(P.x, P.y, P.z, 1.0f);
+                if constexpr (POOL) *reinterpret_cast<float4*>(rec + kRecHead) = make_float4(P.x, P.y, P.z, 1.0f);
             }
         }
 
-        // Queue the step's rays in the wave's pool and trace them together.
-        {
+        TravOut tr{kMiss, 0.0f, false};
+        uint32_t occ = 0;    // NEE lanes: bit k set when the shadow ray of slot k is occluded
+        if constexpr (!POOL) {
+            // One ray per lane, from registers.
+            const TravOut t = traverse<BLOCK, COUNT>(j.sc, o0, d0, t0max, j.eps, phase != kPhTrace, nray != 0u, stk, wc);
+            if (phase == kPhTrace) tr = t;
+            else occ = t.hit ? 1u : 0u;
+        } else {
+            // The wave traces all queued rays together from its pool.  Path
+            // state that tracing does not need waits in the record, so it
+            // holds no registers meanwhile.
+            float4* park = reinterpret_cast<float4*>(rec);
+            park[0] = make_float4(ro.x, ro.y, ro.z, dj.x);
+            park[1] = make_float4(rd.x, rd.y, rd.z, dj.y);
+            park[2] = make_float4(P.x, P.y, P.z, dj.z);
+            park[3] = make_float4(N.x, N.y, N.z, 0.0f);
             const unsigned long long lt = (1ull << __lane_id()) - 1ull;
             uint32_t pre = 0, tot = 0;
             for (uint32_t b = 0; (1u << b) <= kRaySlots; ++b) {   // the bits of nray (<= kRaySlots)
@@ -319,13 +354,6 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
             }
             s_occ[threadIdx.x] = 0u;
             s_pre[threadIdx.x] = pre;
-            // Path state that tracing does not need waits in the record, so it
-            // holds no registers meanwhile.
-            float4* park = reinterpret_cast<float4*>(rec);
-            park[0] = make_float4(ro.x, ro.y, ro.z, dj.x);
-            park[1] = make_float4(rd.x, rd.y, rd.z, dj.y);
-            park[2] = make_float4(P.x, P.y, P.z, dj.z);
-            park[3] = make_float4(N.x, N.y, N.z, 0.0f);
             // The pool, the records and the result words are shared by the
             // wave's lanes: order the writes above before the reads below.
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -341,15 +369,16 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
             P = f3{k2.x, k2.y, k2.z};
             N = f3{k3.x, k3.y, k3.z};
             dj = f3{k0.w, k1.w, k2.w};
+            if (phase == kPhTrace) {
+                const uint2 r = s_res[threadIdx.x];
+                tr.prim = r.x;
+                tr.t = __uint_as_float(r.y);
+                tr.hit = r.x != kMiss;
+            } else {
+                occ = s_occ[threadIdx.x];
+            }
         }
         if (!has) continue;
-        TravOut tr{kMiss, 0.0f, false};
-        if (phase == kPhTrace) {
-            const uint2 r = s_res[threadIdx.x];
-            tr.prim = r.x;
-            tr.t = __uint_as_float(r.y);
-            tr.hit = r.x != kMiss;
-        }
 
         bool fin_diffuse = false, do_rr = false, path_end = false;
         if (phase == kPhTrace) {
@@ -447,11 +476,14 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
                 }
             }
         } else {
-            const uint32_t occ = s_occ[threadIdx.x];
             for (uint32_t k = 0; k < nray; ++k) {
                 if ((occ >> k) & 1u) continue;                       // :473
-                const float4 c = reinterpret_cast<const float4*>(rec + kRecSlots + k * 32u)[1];
-                e = add(e, f3{c.x, c.y, c.z});                       // :502
+                if constexpr (POOL) {
+                    const float4 c = reinterpret_cast<const float4*>(rec + kRecSlots + k * 32u)[1];
+                    e = add(e, f3{c.x, c.y, c.z});                   // :502
+                } else {
+                    e = add(e, c0);
+                }
             }
             if (ei >= j.sc.n_emitters) fin_diffuse = true;
         }
@@ -524,6 +556,20 @@ __global__ __launch_bounds__(BLOCK) void k_path(const PathLaunch A)
         flush_counter(&A.counters->wave_leaves, wc.wave_leaves);
         flush_counter(&A.counters->wave_steps, wave_steps);
     }
+}
+
+// The timed kernel (4 waves per SIMD: at most 128 VGPRs) and the work-counting
+// variant (same traversal order, more registers).
+template <int BLOCK, int STACK, bool BATCH, bool POOL>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_path(const PathLaunch A)
+{
+    path_program<BLOCK, STACK, false, BATCH, POOL>(A);
+}
+
+template <int BLOCK, int STACK, bool BATCH, bool POOL>
+__global__ __launch_bounds__(BLOCK) void k_path_count(const PathLaunch A)
+{
+    path_program<BLOCK, STACK, true, BATCH, POOL>(A);
 }
 
 // ---------------------------------------------------------------------------
@@ -788,35 +834,28 @@ __global__ void k_zero(DevCounters* c, uint32_t* pool)
 // Host launchers.
 // ---------------------------------------------------------------------------
 namespace {
-template <int BLOCK, int STACK, bool COUNT, bool BATCH>
-hipError_t go(const PathLaunch& a, uint32_t grid, hipStream_t s)
+template <int BLOCK, int STACK, bool COUNT, bool BATCH, bool POOL>
+const void* fn()
 {
-    hipLaunchKernelGGL((k_path<BLOCK, STACK, COUNT, BATCH>), dim3(grid), dim3(BLOCK), 0, s, a);
-    return hipGetLastError();
-}
-
-template <int BLOCK, int STACK>
-hipError_t go2(const PathLaunch& a, bool count, bool batch, uint32_t grid, hipStream_t s)
-{
-    if (count) return batch ? go<BLOCK, STACK, true, true>(a, grid, s) : go<BLOCK, STACK, true, false>(a, grid, s);
-    return batch ? go<BLOCK, STACK, false, true>(a, grid, s) : go<BLOCK, STACK, false, false>(a, grid, s);
+    if constexpr (COUNT) return reinterpret_cast<const void*>(&k_path_count<BLOCK, STACK, BATCH, POOL>);
+    else return reinterpret_cast<const void*>(&k_path<BLOCK, STACK, BATCH, POOL>);
 }
 
 template <int BLOCK, int STACK, bool COUNT, bool BATCH>
-const void* fn() { return reinterpret_cast<const void*>(&k_path<BLOCK, STACK, COUNT, BATCH>); }
+const void* fn_pool(bool pool) { return pool ? fn<BLOCK, STACK, COUNT, BATCH, true>() : fn<BLOCK, STACK, COUNT, BATCH, false>(); }
 
 template <int BLOCK, int STACK>
-const void* fn2(bool count, bool batch)
+const void* fn_flags(const PathVariant& v)
 {
-    if (count) return batch ? fn<BLOCK, STACK, true, true>() : fn<BLOCK, STACK, true, false>();
-    return batch ? fn<BLOCK, STACK, false, true>() : fn<BLOCK, STACK, false, false>();
+    if (v.count) return v.batch ? fn_pool<BLOCK, STACK, true, true>(v.pool) : fn_pool<BLOCK, STACK, true, false>(v.pool);
+    return v.batch ? fn_pool<BLOCK, STACK, false, true>(v.pool) : fn_pool<BLOCK, STACK, false, false>(v.pool);
 }
 
-const void* kernel_ptr(uint32_t block, uint32_t stack, bool count, bool batch)
+const void* kernel_ptr(const PathVariant& v)
 {
-    if (block == 64) return stack <= 32 ? fn2<64, 32>(count, batch) : fn2<64, 64>(count, batch);
-    if (block == 128) return stack <= 32 ? fn2<128, 32>(count, batch) : fn2<128, 64>(count, batch);
-    return stack <= 32 ? fn2<256, 32>(count, batch) : fn2<256, 64>(count, batch);
+    if (v.block == 64) return v.stack <= 32 ? fn_flags<64, 32>(v) : fn_flags<64, 64>(v);
+    if (v.block == 128) return v.stack <= 32 ? fn_flags<128, 32>(v) : fn_flags<128, 64>(v);
+    return v.stack <= 32 ? fn_flags<256, 32>(v) : fn_flags<256, 64>(v);
 }
 }  // namespace
 
@@ -827,35 +866,32 @@ uint32_t stack_variant(uint32_t needed)
     return 0;
 }
 
-hipError_t launch_path(const PathLaunch& a, uint32_t block, uint32_t stack, bool count, bool batch, uint32_t grid,
-                       hipStream_t s)
+hipError_t launch_path(const PathLaunch& a, const PathVariant& v, uint32_t grid, hipStream_t s)
 {
-    if (block == 64) return stack <= 32 ? go2<64, 32>(a, count, batch, grid, s) : go2<64, 64>(a, count, batch, grid, s);
-    if (block == 128)
-        return stack <= 32 ? go2<128, 32>(a, count, batch, grid, s) : go2<128, 64>(a, count, batch, grid, s);
-    return stack <= 32 ? go2<256, 32>(a, count, batch, grid, s) : go2<256, 64>(a, count, batch, grid, s);
+    void* args[] = {const_cast<PathLaunch*>(&a)};
+    return hipLaunchKernel(kernel_ptr(v), dim3(grid), dim3(v.block), args, 0, s);
 }
 
-uint32_t persistent_grid(uint32_t block, uint32_t stack, bool count, bool batch, int device)
+uint32_t persistent_grid(const PathVariant& v, int device)
 {
     // Resident workgroups per CU x CUs, cached per (device, variant).
-    struct Entry { int device; uint32_t block, stack; bool count, batch; uint32_t grid; };
-    static thread_local Entry cache[32];
+    struct Entry { int device; PathVariant v; uint32_t grid; };
+    static thread_local Entry cache[64];
     static thread_local int n = 0;
     for (int i = 0; i < n; ++i) {
         const Entry& c = cache[i];
-        if (c.device == device && c.block == block && c.stack == stack && c.count == count && c.batch == batch)
+        if (c.device == device && c.v.block == v.block && c.v.stack == v.stack && c.v.count == v.count &&
+            c.v.batch == v.batch && c.v.pool == v.pool)
             return c.grid;
     }
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(block, stack, count, batch), (int)block,
-                                                     0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(v), (int)v.block, 0) != hipSuccess ||
         per_cu < 1)
         per_cu = 1;
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 256;
     const uint32_t grid = (uint32_t)(per_cu * cus);
-    if (n < 32) cache[n++] = Entry{device, block, stack, count, batch, grid};
+    if (n < 64) cache[n++] = Entry{device, v, grid};
     return grid;
 }
 

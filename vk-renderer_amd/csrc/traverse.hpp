@@ -286,22 +286,26 @@ __device__ __forceinline__ bool leave_nodes(uint32_t nn, uint32_t nw)
     return nn == 0 || (nw > 0 && nw * 8u >= (nw + nn) * kLeafExit8);
 }
 
-// One ray per lane (the adjoint kernel's replay).
+// One ray per lane (lanes with active == false only take part in the
+// wave's loop).
 template <int BLOCK, bool COUNT>
 __device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, float tmax, float eps, bool any,
-                                            uint32_t* stk, WorkCnt& wc)
+                                            bool active, uint32_t* stk, WorkCnt& wc)
 {
     Ray r;
     r.start(sc, o, d, tmax, eps, any);
-    while (true) {
+    if (!active) r.ref = kDone;
+    if (r.ref != kDone) {   // (only traversing lanes in the loop: 11 fewer VGPRs in k_path)
         while (true) {
-            const uint32_t nn = (uint32_t)__popcll(__ballot((int)r.ref >= 0));
-            const uint32_t nl = (uint32_t)__popcll(__ballot((int)r.ref < -1));
-            if (leave_nodes(nn, nl)) break;
-            if ((int)r.ref >= 0) r.node<BLOCK, COUNT>(sc.nodes, stk, wc);
+            while (true) {
+                const uint32_t nn = (uint32_t)__popcll(__ballot((int)r.ref >= 0));
+                const uint32_t nl = (uint32_t)__popcll(__ballot((int)r.ref < -1));
+                if (leave_nodes(nn, nl)) break;
+                if ((int)r.ref >= 0) r.node<BLOCK, COUNT>(sc.nodes, stk, wc);
+            }
+            if (__ballot(r.ref != kDone) == 0) break;
+            if ((int)r.ref < -1) r.leaf<BLOCK, COUNT>(sc, eps, stk, wc);
         }
-        if (__ballot(r.ref != kDone) == 0) break;
-        if ((int)r.ref < -1) r.leaf<BLOCK, COUNT>(sc, eps, stk, wc);
     }
     return r.finish<COUNT>(sc, eps, wc);
 }

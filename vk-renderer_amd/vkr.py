@@ -39,6 +39,7 @@ DIFFUSE, DIELECTRIC = 0, 1
 KERNEL_PATH, KERNEL_WHITTED = 0, 1
 SCHED_AUTO, SCHED_PLAIN, SCHED_PERSISTENT = 0, 1, 2
 TRACE_RECORD_DECISIONS, TRACE_REPLAY_DECISIONS = 1, 2
+TRACE_RAY_POOL, TRACE_NO_RAY_POOL = 4, 8
 SCENE_TRI_MATERIAL_INDEX = 1
 CTX_COUNT_WORK = 1
 DEBUG_OPS = {"rand": 0, "tri": 1, "sphere": 2, "plane": 3, "brdf": 4, "jitter": 5, "cos": 6}
