@@ -163,7 +163,7 @@ typedef struct vkr_scene_desc {
     const vkr_material* tri_material; /* host; NULL = the reference `mirror` (Tracer.comp:192) */
     const vkr_material* materials;   uint32_t n_materials; /* host; per-triangle table */
     uint32_t flags;                  /* VKR_SCENE_* */
-    uint32_t max_leaf_size;          /* BVH leaf size cap (0 = 2; at most 8) */
+    uint32_t max_leaf_size;          /* BVH leaf size cap (0 = 1 with 2+ emitters, else 2; at most 8) */
     uint32_t max_depth;              /* BVH depth cap (0 = 32; at most 64) */
     float    traversal_cost;         /* SAH cost of a node visit relative to one primitive test (0 = 3) */
 } vkr_scene_desc;

@@ -70,6 +70,34 @@ def test_batch_matches_single_frames(ctx, torch_cuda):
                 assert torch.equal(s[k], b[k]), (block, chunk, k)
 
 
+def test_batch_mixed_jobs_in_waves(ctx, torch_cuda):
+    """Eight multi-emitter scenes of very different sizes (pooled kernel), small
+    images and a refill threshold of 1: waves hold pixels of several jobs and
+    trace each other's shadow rays, each against its own job's scene."""
+    torch = torch_cuda
+    wls = []
+    for i, n in enumerate((200, 40000, 1000, 12000, 300, 60000, 5000, 800)):
+        wl = scenes.soup100k(seed=80 + i, n_tris=n, width=24 + 8 * (i % 3), height=16 + 8 * (i % 2), spp=2,
+                             max_depth=4, n_spheres=24)
+        wls.append(wl)
+    sids = [ctx.upload(wl.scene) for wl in wls]
+    assert all(ctx.info(sid)["n_emitters"] >= 2 for sid in sids)
+    single = []
+    for wl, sid in zip(wls, sids):
+        b = _bufs(torch, wl)
+        ctx.trace(sid, wl.frame, wl.params(), **b)
+        single.append(b)
+    torch.cuda.synchronize()
+    for block, chunk in ((64, 1), (128, 1), (256, 4)):
+        batched = [_bufs(torch, wl) for wl in wls]
+        jobs = [ctx.job(sid, wl.frame, wl.params(), **b) for wl, sid, b in zip(wls, sids, batched)]
+        ctx.trace_batch(jobs, block=block, chunk=chunk)
+        torch.cuda.synchronize()
+        for s, b in zip(single, batched):
+            for k in KEYS:
+                assert torch.equal(s[k], b[k]), (block, chunk, k)
+
+
 def test_graph_replays_match_eager(ctx, torch_cuda):
     torch = torch_cuda
     wls = _small_jobs()[:3]

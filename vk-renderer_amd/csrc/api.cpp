@@ -100,6 +100,8 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
     VKR_HIP(hipSetDevice(ctx->device));
 
     auto sc = std::make_unique<Scene>();
+    sc->host_spheres.assign(desc->spheres, desc->spheres + desc->n_spheres);
+    const std::vector<uint32_t> em = emitter_list(sc->host_spheres);
     const auto t0 = std::chrono::steady_clock::now();
     BuildInput bi;
     bi.tri_v = reinterpret_cast<const float*>(desc->tris);
@@ -107,7 +109,7 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
     bi.n_tris = desc->n_tris;
     bi.sph = reinterpret_cast<const float*>(desc->spheres);
     bi.n_spheres = desc->n_spheres;
-    bi.max_leaf = desc->max_leaf_size ? desc->max_leaf_size : kDefaultLeaf;
+    bi.max_leaf = desc->max_leaf_size ? desc->max_leaf_size : default_leaf((uint32_t)em.size());
     bi.max_depth = desc->max_depth ? desc->max_depth : 32u;
     bi.traversal_cost = desc->traversal_cost;
     BuildOutput bo = build_bvh(bi);
@@ -118,8 +120,6 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
     if (bo.nodes.size() >= (1u << 26))
         return fail(VKR_UNSUPPORTED, "BVH has 2^26 nodes or more (nodes are addressed with 32-bit offsets)");
 
-    sc->host_spheres.assign(desc->spheres, desc->spheres + desc->n_spheres);
-    const std::vector<uint32_t> em = emitter_list(sc->host_spheres);
     // Parameters as uploaded: light scale 1, emission = the sphere's emissive.
     sc->host_theta.assign(7u * desc->n_spheres + 1u, 0.0f);
     for (uint32_t i = 0; i < desc->n_spheres; ++i) {

@@ -42,9 +42,13 @@ constexpr uint32_t kRefCountShift = 27;
 constexpr uint32_t kRefFirstMask = (1u << 27) - 1u;
 constexpr uint32_t kMaxLeaf = 8;
 // Default leaf size cap: small leaves keep the lanes of a wave's leaf pass
-// busy (on the incoherent 1080p / 100k-triangle frame, caps of 1-2 run ~9%
-// faster than 4, at 1.8x the nodes of 4).
+// busy.  Scenes with several emitters are traced by the pooled kernel,
+// mostly shadow rays: there single-primitive leaves run fastest (1080p /
+// 100k-triangle soup: 113 ms against 122 ms with 2, 135 ms with 3).  With
+// one emitter, 2 is best on the 2M-triangle heightfield (9.9 ms against
+// 10.3 ms with 1).
 constexpr uint32_t kDefaultLeaf = 2;
+inline uint32_t default_leaf(uint32_t n_emitters) { return n_emitters >= 2u ? 1u : kDefaultLeaf; }
 
 struct alignas(16) BvhNode {
     float a[4], b[4], c[4];

@@ -69,8 +69,9 @@ constexpr uint32_t kMaxPlainLanes = 1u << 21;
 // Per-lane ray record of the path kernel (one per lane of a launch, in
 // global memory):
 //   [0, 64)    path state parked while the wave traces
-//   [64, 80)   this step's ray origin xyz, w = 1.0 for shadow rays, 0.0 for a
-//              closest-hit ray
+//   [64, 80)   this step's ray origin xyz; w holds the bits of
+//              (job << 1) | 1 for shadow rays, (job << 1) for a closest-hit
+//              ray (job: index in the batch, 0 for a single frame)
 //   [80, ...)  kRaySlots slots of 32 bytes: (d xyz, t_max), (next-event
 //              contribution xyz, 0)
 constexpr uint32_t kRaySlots = 32;

@@ -160,7 +160,8 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
                 const float dp1 = (float)(depth + 1u);
                 const float tmax = 3000.0f / (dp1 * dp1);            // :444
                 if constexpr (POOL) {
-                    *reinterpret_cast<float4*>(rec + kRecHead) = make_float4(ro.x, ro.y, ro.z, 0.0f);
+                    *reinterpret_cast<float4*>(rec + kRecHead) =
+                        make_float4(ro.x, ro.y, ro.z, __uint_as_float(BATCH ? job << 1 : 0u));
                     *reinterpret_cast<float4*>(rec + kRecSlots) = make_float4(rd.x, rd.y, rd.z, tmax);
                 } else {
                     o0 = ro;
@@ -325,7 +326,9 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
                         }
                     }
                 }
-                if constexpr (POOL) *reinterpret_cast<float4*>(rec + kRecHead) = make_float4(P.x, P.y, P.z, 1.0f);
+                if constexpr (POOL)
+                    *reinterpret_cast<float4*>(rec + kRecHead) =
+                        make_float4(P.x, P.y, P.z, __uint_as_float(BATCH ? (job << 1) | 1u : 1u));
             }
         }
 
@@ -359,7 +362,8 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            traverse_pool<BLOCK, COUNT>(j.sc, j.eps, tot, s_pre + wbase, s_occ + wbase, s_res + wbase, wrec, stk, wc);
+            traverse_pool<BLOCK, COUNT, BATCH>(BATCH ? A.jobs : &A.job, tot, s_pre + wbase, s_occ + wbase,
+                                               s_res + wbase, wrec, stk, wc);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

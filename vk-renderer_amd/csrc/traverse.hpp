@@ -318,11 +318,14 @@ constexpr uint32_t kNone = 0xFFFFu;
 // rays' tail instead of its longest lane.  A shadow ray sets bit `slot` of
 // occ[owner] when occluded; the closest-hit ray of lane `owner` writes
 // (prim, t) to res[owner].  `wrec` is the record of the wave's lane 0.
-template <int BLOCK, bool COUNT>
-__device__ __forceinline__ void traverse_pool(const DevScene& sc, float eps, uint32_t total, const uint32_t* pre,
+template <int BLOCK, bool COUNT, bool BATCH>
+__device__ __forceinline__ void traverse_pool(const JobDesc* jobs, uint32_t total, const uint32_t* pre,
                                               uint32_t* occ, uint2* res, const char* __restrict__ wrec,
                                               uint32_t* stk, WorkCnt& wc)
 {
+    // The job (scene, epsilon) of the ray in flight: a batched wave can hold
+    // pixels of several jobs, and a lane may trace another lane's ray.
+    const JobDesc* rj = jobs;
     Ray r;
     r.ref = kDone;
     uint32_t cur = kNone;
@@ -332,7 +335,7 @@ __device__ __forceinline__ void traverse_pool(const DevScene& sc, float eps, uin
         // or the pool is empty.
         while (true) {
             if (cur != kNone && r.ref == kDone) {
-                const TravOut t = r.finish<COUNT>(sc, eps, wc);
+                const TravOut t = r.finish<COUNT>(rj->sc, rj->eps, wc);
                 const uint32_t owner = cur >> 5;
                 if (r.any) {
                     if (t.hit) atomicOr(occ + owner, 1u << (cur & 31u));
@@ -356,7 +359,9 @@ __device__ __forceinline__ void traverse_pool(const DevScene& sc, float eps, uin
                     const char* rec = wrec + (cur >> 5) * kLaneBytes;
                     const float4 h = *reinterpret_cast<const float4*>(rec + kRecHead);
                     const float4 s = *reinterpret_cast<const float4*>(rec + kRecSlots + (cur & 31u) * 32u);
-                    r.start(sc, f3{h.x, h.y, h.z}, f3{s.x, s.y, s.z}, s.w, eps, h.w != 0.0f);
+                    const uint32_t hw = __float_as_uint(h.w);   // (owner's job << 1) | shadow ray
+                    if constexpr (BATCH) rj = jobs + (hw >> 1);
+                    r.start(rj->sc, f3{h.x, h.y, h.z}, f3{s.x, s.y, s.z}, s.w, rj->eps, (hw & 1u) != 0u);
                 }
                 next += (uint32_t)__popcll(want);
             }
@@ -368,9 +373,9 @@ __device__ __forceinline__ void traverse_pool(const DevScene& sc, float eps, uin
             const uint32_t nn = (uint32_t)__popcll(__ballot((int)r.ref >= 0));
             const uint32_t nw = (uint32_t)__popcll(__ballot((int)r.ref < -1 || (more && r.ref == kDone)));
             if (leave_nodes(nn, nw)) break;
-            if ((int)r.ref >= 0) r.node<BLOCK, COUNT>(sc.nodes, stk, wc);
+            if ((int)r.ref >= 0) r.node<BLOCK, COUNT>(rj->sc.nodes, stk, wc);
         }
-        if ((int)r.ref < -1) r.leaf<BLOCK, COUNT>(sc, eps, stk, wc);
+        if ((int)r.ref < -1) r.leaf<BLOCK, COUNT>(rj->sc, rj->eps, stk, wc);
     }
 }
 
