@@ -73,11 +73,19 @@ constexpr uint32_t kMaxPlainLanes = 1u << 21;
 //              (job << 1) | 1 for shadow rays, (job << 1) for a closest-hit
 //              ray (job: index in the batch, 0 for a single frame)
 //   [80, ...)  kRaySlots slots of 32 bytes: (d xyz, t_max), (next-event
-//              contribution xyz, 0)
+//              contribution xyz, 0); slot k holds emitter ei0 + k of a
+//              next-event window, or slot 0 the path segment
+//   kRecHit    the diffuse hit published for the wave's next-event
+//              evaluation: (N xyz, u), (albedo xyz, roughness),
+//              (metalness, v, bits of ei0, bits of the kNee* flags)
+//   kRecSalt   kRaySlots floats: the random stream position before each
+//              emitter's sample
 constexpr uint32_t kRaySlots = 32;
 constexpr uint32_t kRecHead = 64u;
 constexpr uint32_t kRecSlots = 80u;
-constexpr uint32_t kLaneBytes = kRecSlots + 32u * kRaySlots;
+constexpr uint32_t kRecHit = kRecSlots + 32u * kRaySlots;
+constexpr uint32_t kRecSalt = kRecHit + 48u;
+constexpr uint32_t kLaneBytes = kRecSalt + 4u * kRaySlots;
 
 struct PathLaunch {
     JobDesc job;                  // single-frame launches

@@ -28,6 +28,132 @@
 
 namespace vkr {
 
+// A diffuse hit as next-event estimation sees it.
+constexpr uint32_t kNeeZero = 1u;     // every emitter contributes exactly +-0 (mirror, |N|^2 < 0.9999)
+constexpr uint32_t kNeeFin = 2u;      // the BRDF factor is bounded for any sample
+constexpr uint32_t kNeeMirror = 4u;   // metalness 1, roughness 0: zero unless (N.H)^2 rounds to 1
+struct NeeHit {
+    f3 P, N, V, alb;
+    float rough, metal;
+    uint32_t flags;                   // kNee*
+};
+
+// The next-event sample of emitter `ei` at hit h (Assets/Tracer.comp:464-502)
+// with the random stream at `salt` before its two draws (the caller advances
+// its own copy).  Returns false when the contribution is exactly +-0: then
+// adding it cannot change e whatever the shadow ray finds, so the ray is not
+// traced.  Else L is the shadow ray's direction, tl its length and c the
+// contribution added when it is unoccluded.
+__device__ __forceinline__ bool nee_sample(const JobDesc& j, const NeeHit& h, uint32_t ei, float salt, float uvx,
+                                           float uvy, f3& L, float& tl, f3& c)
+{
+    const vkr_sphere* S = j.sc.spheres + j.sc.emitters[ei];
+    const f3 P = h.P, N = h.N, V = h.V;
+    // Emitters certified to contribute exactly +-0 before sampling need no
+    // draws.  The tests carry margins far above the rounding of the fast
+    // reciprocal / square-root estimates they use; an emitter they cannot
+    // certify is evaluated in full.
+    auto certified = [&]() __attribute__((always_inline)) -> bool {
+        const float nn = dot(N, N);
+        const f3 l0 = sub(ld3(S->P), P);
+        const float sr = S->r;
+        const float ll2 = dot(l0, l0);
+        // Attenuation < 2^41: the hit point is at least 2^-9 radii from the centre.
+        if (!(ll2 >= (sr * sr) * 0x1p-18f)) return false;
+        // A sample L has cosa >= cos_a_max and sina <= sig, sig^2 = s2 + 1e-6
+        // >= the kernel's 1 - cos_a_max^2, so it lies within t = 1.5708 sig of
+        // w = l0 / |l0|.
+        const float il = __builtin_amdgcn_rsqf(ll2);
+        const float sw = sr * il;
+        const float sig2 = fminf(sw * sw + 1e-6f, 1.0f);
+        const float sig = __builtin_amdgcn_sqrtf(sig2) * (1.0f + 1e-6f);
+        const float t = 1.5708f * sig;
+        // Every certificate needs the Fresnel term finite: with V at least
+        // 0.01 rad outside the cone, dot(H, V) <= 1 - 1e-5, so
+        // pow(1 - dot(H, V), 5) never sees a negative base (NaN, :276).
+        const f3 w = muls(l0, il);
+        const float cph = dot(V, w);
+        const float tv = t + 0.01f;
+        if (!(tv < 1.5f && cph < 1.0f - 0.5f * tv * tv - 1e-6f)) return false;
+        if (h.flags & kNeeZero) return true;
+        if (h.flags & kNeeFin) {
+            // (b) cone below the horizon: with c = cos(N, l0) < 0 and
+            // s = sin(N, l0) (from |N x l0|, no cancellation),
+            // cos(N, L) <= sig s + cl c < -1e-4.
+            const float inv = __builtin_amdgcn_rsqf(nn * ll2);
+            const float cc = dot(N, l0) * inv;
+            const f3 cr = cross(N, l0);
+            const float ss = __builtin_amdgcn_sqrtf(dot(cr, cr)) * inv;
+            const float cl = __builtin_amdgcn_sqrtf(1.0f - sig2) * (1.0f - 1e-6f);
+            if (cc < 0.0f && sig * ss + cl * cc < -1e-4f) return true;
+        }
+        if (h.flags & kNeeMirror) {
+            // (c) mirror hits: H = normalize(V + L) stays within
+            // dl >= t / (2 cos((phi0 + t) / 2)) of H0 = normalize(V + w)
+            // (|dH| <= |dL| / |V + L|; phi0 = angle(V, w) < 107.5 deg, t < 1,
+            // so the path never nears V + L = 0).  If |N| cos(N, H) then stays
+            // at least 2e-4 away from 1, NdotH^2 cannot round to 1 and the
+            // sample is +-0.
+            const f3 vw = cross(V, w);
+            const float sph = __builtin_amdgcn_sqrtf(dot(vw, vw));
+            const float csum = (cph >= 0.0f ? cph * (1.0f - 0.5f * t * t) : cph) - sph * t;
+            const float half = __builtin_amdgcn_sqrtf(fmaxf(0.5f * (1.0f + csum), 0.0f));
+            if (cph > -0.3f && half >= 0.05f && t < 1.0f) {
+                const float dl = t * __builtin_amdgcn_rcpf(2.0f * half) * 1.001f + 1e-5f;
+                const f3 hv = add(V, w);
+                const float hh = dot(hv, hv);
+                const float ihn = __builtin_amdgcn_rsqf(hh * nn);
+                const float c0 = dot(N, hv) * ihn;
+                const f3 nh = cross(N, hv);
+                const float s0 = __builtin_amdgcn_sqrtf(dot(nh, nh)) * ihn;
+                const float nrm = __builtin_amdgcn_sqrtf(nn);
+                const float cd = 1.0f - 0.5f * dl * dl;
+                const float hi = c0 >= cd - 1e-4f ? 1.0f : (c0 >= 0.0f ? c0 : c0 * cd) + s0 * dl;
+                const float lo = (c0 >= 0.0f ? c0 * cd : c0) - s0 * dl;
+                if (dl < 1.0f && hh > 1e-6f && (nrm * hi < 1.0f - 2e-4f || nrm * lo > 1.0f + 2e-4f)) return true;
+            }
+        }
+        if (h.flags & kNeeFin) {
+            // (a) attenuation exactly 0 (:478-479): with the kernel's own x,
+            // emis / x^2 <= kAttSub in every channel, tested with a 2^-16 margin.
+            const float tl0 = length(l0) - sr;
+            const float x = tl0 / sr + 1.0f;
+            const float lim = (kAttSub * (x * x)) * (1.0f - 0x1p-16f);
+            const f3 em = ld3(S->mat.emissive);
+            if (em.x <= lim && em.y <= lim && em.z <= lim) return true;
+        }
+        return false;
+    };
+    if (h.flags && certified()) return false;
+
+    const f3 sP = ld3(S->P);
+    const float sr = S->r;
+    tl = length(sub(sP, P)) - sr;
+    const f3 l0 = sub(sP, P);
+    const float cos_a_max = sqrtf(1.0f - gclamp(sr * sr / dot(l0, l0), 0.0f, 1.0f));
+    const float cosa = gmix(cos_a_max, 1.0f, rng_next(salt, uvx, uvy));
+    const float phi = kTwoPi * rng_next(salt, uvx, uvy);
+    L = jitter(l0, phi, sqrtf(1.0f - cosa * cosa), cosa);
+
+    const float x = tl / sr + 1.0f;
+    f3 att = divs(muls(ld3(S->mat.emissive), 1.0f), x * x);
+    att = divs(subs(att, kAttSub), kAttDiv);
+    att = maxs(att, 0.0f);
+    const f3 F0 = mix3(f3{0.04f, 0.04f, 0.04f}, h.alb, h.metal);
+    const f3 H = normalize(add(V, L));
+    const float NDF = distribution_ggx(N, H, h.rough);
+    const float G = geometry_smith(N, V, L, h.rough);
+    const f3 F = fresnel_schlick(gmax(dot(H, V), 0.0f), F0);
+    f3 kD = ssub(1.0f, F);
+    kD = muls(kD, 1.0f - h.metal);
+    const f3 numerator = muls(F, NDF * G);
+    const float denominator = 4.0f * gmax(dot(N, V), 0.0f) * gmax(dot(N, L), 0.0f);
+    const f3 specular = divs(numerator, gmax(denominator, 0.001f));
+    const float NdotL = gmax(dot(N, L), 0.0f);
+    c = muls(mul(add(divs(mul(kD, h.alb), kPi), specular), att), NdotL);
+    return !(c.x == 0.0f && c.y == 0.0f && c.z == 0.0f);
+}
+
 // The path program of Assets/Tracer.comp:433-593 as a per-lane state machine.
 // Each loop iteration: (1) refill idle lanes, (2) queue this lane's rays
 // (a path segment, or the shadow rays of the next emitters), (3) trace the
@@ -41,6 +167,7 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
     __shared__ uint32_t s_pre[BLOCK];                // per lane: index of its first queued ray in the wave
     __shared__ uint32_t s_occ[BLOCK];                // per lane: occluded shadow rays (bit per slot)
     __shared__ uint2 s_res[BLOCK];                   // per lane: closest hit (prim, t)
+    __shared__ uint32_t s_msk[BLOCK];                // per lane: slots holding a ray to trace
     uint32_t* stk = s_stack + threadIdx.x;
     const uint32_t wbase = __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
     char* const wrec = A.lanes + ((size_t)blockIdx.x * BLOCK + wbase) * kLaneBytes;
@@ -60,9 +187,7 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
     float t0 = 0.0f;
     uint32_t c_rays = 0, c_shadow = 0, c_traced = 0, wave_steps = 0;
     bool new_path = false;
-    bool nee_zero = false;              // every emitter of this hit contributes exactly +-0
-    bool nee_fin = false;               // the BRDF factor of this hit is finite (bounded) for any sample
-    bool nee_mirror = false;            // metalness 1, roughness 0: zero unless (N.H)^2 rounds to 1
+    uint32_t nee_flags = 0;             // kNee* of the current diffuse hit
     WorkCnt wc{};
 
     // Claims pool index g for this lane (no-op for indices outside the image).
@@ -152,6 +277,7 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
         // else at most one, kept in registers (o0, d0, t0max; c0 = its
         // next-event contribution).
         uint32_t nray = 0;
+        uint32_t nitem = 0;  // POOL: emitters this lane published for the wave to evaluate
         f3 o0{}, d0{}, c0{};
         float t0max = 0.0f;
         if (has) {
@@ -174,161 +300,128 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
                 ++c_traced;
             } else {
                 // Next-event samples (:464-469) and their unoccluded
-                // contributions (:475-502), evaluated before the shadow rays.
-                // A contribution of exactly +-0 cannot change e whatever the
-                // ray's outcome, so that ray is counted but not traversed.
-                // Rays that must be traced are queued with their
-                // contributions (up to kRaySlots per step with POOL, else
-                // one) and added to e in emitter order once their outcomes
-                // are known: the same additions in the same order as the
-                // reference's loop.
-                const f3 alb = ld3(hm->albedo);
-                const float rough = hm->roughness, metal = hm->metalness;
-                const f3 V = normalize(sub(ld3(j.cam_pos), P));
-                while (ei < j.sc.n_emitters) {
-                    if (nee_zero || nee_fin || nee_mirror) {
-                        // Emitters certified to contribute exactly +-0 before
-                        // sampling are counted, and their two random draws
-                        // (:464-465) skipped by advancing the stream state
-                        // alone.  The tests carry margins far above the
-                        // rounding of the fast reciprocal / square-root
-                        // estimates they use; any emitter they cannot certify
-                        // is evaluated in full below.
-                        const float nn = dot(N, N);
-                        while (ei < j.sc.n_emitters) {
-                            const vkr_sphere* S = j.sc.spheres + j.sc.emitters[ei];
-                            const f3 l0 = sub(ld3(S->P), P);
-                            const float sr = S->r;
-                            const float ll2 = dot(l0, l0);
-                            // Attenuation < 2^41: the hit point is at least 2^-9
-                            // radii from the centre.
-                            if (!(ll2 >= (sr * sr) * 0x1p-18f)) break;
-                            // A sample L has cosa >= cos_a_max and sina <= sig,
-                            // sig^2 = s2 + 1e-6 >= the kernel's 1 - cos_a_max^2, so
-                            // it lies within t = 1.5708 sig of w = l0 / |l0|.
-                            const float il = __builtin_amdgcn_rsqf(ll2);
-                            const float sw = sr * il;
-                            const float sig2 = fminf(sw * sw + 1e-6f, 1.0f);
-                            const float sig = __builtin_amdgcn_sqrtf(sig2) * (1.0f + 1e-6f);
-                            const float t = 1.5708f * sig;
-                            // Every certificate needs the Fresnel term finite: with
-                            // V at least 0.01 rad outside the cone, dot(H, V) <= 1 - 1e-5,
-                            // so pow(1 - dot(H, V), 5) never sees a negative base
-                            // (NaN, :276).
-                            const f3 w = muls(l0, il);
-                            const float cph = dot(V, w);
-                            const float tv = t + 0.01f;
-                            if (!(tv < 1.5f && cph < 1.0f - 0.5f * tv * tv - 1e-6f)) break;
-                            bool zero = nee_zero;
-                            if (!zero && nee_fin) {
-                                // (b) cone below the horizon: with c = cos(N, l0)
-                                // < 0 and s = sin(N, l0) (from |N x l0|, no
-                                // cancellation), cos(N, L) <= sig s + cl c < -1e-4.
-                                const float inv = __builtin_amdgcn_rsqf(nn * ll2);
-                                const float c = dot(N, l0) * inv;
-                                const f3 cr = cross(N, l0);
-                                const float s = __builtin_amdgcn_sqrtf(dot(cr, cr)) * inv;
-                                const float cl = __builtin_amdgcn_sqrtf(1.0f - sig2) * (1.0f - 1e-6f);
-                                zero = c < 0.0f && sig * s + cl * c < -1e-4f;
-                            }
-                            if (!zero && nee_mirror) {
-                                // (c) mirror hits: H = normalize(V + L) stays within
-                                // dl >= t / (2 cos((phi0 + t) / 2)) of H0 = normalize(V + w)
-                                // (|dH| <= |dL| / |V + L|; phi0 = angle(V, w) < 107.5 deg,
-                                // t < 1, so the path never nears V + L = 0).  If
-                                // |N| cos(N, H) then stays at least 2e-4 away from 1,
-                                // NdotH^2 cannot round to 1 and the sample is +-0.
-                                const f3 vw = cross(V, w);
-                                const float sph = __builtin_amdgcn_sqrtf(dot(vw, vw));
-                                const float csum = (cph >= 0.0f ? cph * (1.0f - 0.5f * t * t) : cph) - sph * t;
-                                const float half = __builtin_amdgcn_sqrtf(fmaxf(0.5f * (1.0f + csum), 0.0f));
-                                if (cph > -0.3f && half >= 0.05f && t < 1.0f) {
-                                    const float dl = t * __builtin_amdgcn_rcpf(2.0f * half) * 1.001f + 1e-5f;
-                                    const f3 h = add(V, w);
-                                    const float hh = dot(h, h);
-                                    const float ihn = __builtin_amdgcn_rsqf(hh * nn);
-                                    const float c0 = dot(N, h) * ihn;
-                                    const f3 nh = cross(N, h);
-                                    const float s0 = __builtin_amdgcn_sqrtf(dot(nh, nh)) * ihn;
-                                    const float nrm = __builtin_amdgcn_sqrtf(nn);
-                                    const float cd = 1.0f - 0.5f * dl * dl;
-                                    const float hi =
-                                        c0 >= cd - 1e-4f ? 1.0f : (c0 >= 0.0f ? c0 : c0 * cd) + s0 * dl;
-                                    const float lo = (c0 >= 0.0f ? c0 * cd : c0) - s0 * dl;
-                                    zero = dl < 1.0f && hh > 1e-6f &&
-                                           (nrm * hi < 1.0f - 2e-4f || nrm * lo > 1.0f + 2e-4f);
-                                }
-                            }
-                            if (!zero && nee_fin) {
-                                // (a) attenuation exactly 0 (:478-479): with the
-                                // kernel's own x, emis / x^2 <= kAttSub in every
-                                // channel, tested with a 2^-16 margin.
-                                const float tl = length(l0) - sr;
-                                const float x = tl / sr + 1.0f;
-                                const float lim = (kAttSub * (x * x)) * (1.0f - 0x1p-16f);
-                                const f3 em = ld3(S->mat.emissive);
-                                zero = em.x <= lim && em.y <= lim && em.z <= lim;
-                            }
-                            if (!zero) break;
-                            salt = salt + 1.0f;      // the sample's two draws (:464-465)
-                            salt = salt + 1.0f;
-                            ++rays_px;
-                            ++c_rays;
-                            ++c_shadow;
-                            ++ei;
-                        }
-                        if (ei >= j.sc.n_emitters) break;
+                // contributions (:475-502), evaluated before the shadow rays
+                // (nee_sample).  Rays that must be traced are added to e in
+                // emitter order once their outcomes are known: the same
+                // additions in the same order as the reference's loop.
+                NeeHit h;
+                h.P = P;
+                h.N = N;
+                h.V = normalize(sub(ld3(j.cam_pos), P));
+                h.alb = ld3(hm->albedo);
+                h.rough = hm->roughness;
+                h.metal = hm->metalness;
+                h.flags = nee_flags;
+                if constexpr (POOL) {
+                    // Publish the hit and its next window of up to kRaySlots
+                    // emitters; the wave evaluates all (lane, emitter) pairs
+                    // together below.  Each sample draws twice (:464-465), so
+                    // the stream position before each one is known now.
+                    const uint32_t w = min(kRaySlots, j.sc.n_emitters - ei);
+                    float4* hr = reinterpret_cast<float4*>(rec + kRecHit);
+                    hr[0] = make_float4(N.x, N.y, N.z, uvx);
+                    hr[1] = make_float4(h.alb.x, h.alb.y, h.alb.z, h.rough);
+                    hr[2] = make_float4(h.metal, uvy, __uint_as_float(ei), __uint_as_float(nee_flags));
+                    float* sk = reinterpret_cast<float*>(rec + kRecSalt);
+                    float sv = salt;
+                    for (uint32_t k = 0; k < w; ++k) {
+                        sk[k] = sv;
+                        sv = sv + 1.0f;
+                        sv = sv + 1.0f;
                     }
-                    const vkr_sphere* S = j.sc.spheres + j.sc.emitters[ei];
-                    const f3 sP = ld3(S->P);
-                    const float sr = S->r;
-                    const float tl = length(sub(sP, P)) - sr;
-                    const f3 l0 = sub(sP, P);
-                    const float cos_a_max = sqrtf(1.0f - gclamp(sr * sr / dot(l0, l0), 0.0f, 1.0f));
-                    const float cosa = gmix(cos_a_max, 1.0f, rng_next(salt, uvx, uvy));
-                    const float phi = kTwoPi * rng_next(salt, uvx, uvy);
-                    const f3 L = jitter(l0, phi, sqrtf(1.0f - cosa * cosa), cosa);
-
-                    const float x = tl / sr + 1.0f;
-                    f3 att = divs(muls(ld3(S->mat.emissive), 1.0f), x * x);
-                    att = divs(subs(att, kAttSub), kAttDiv);
-                    att = maxs(att, 0.0f);
-                    const f3 F0 = mix3(f3{0.04f, 0.04f, 0.04f}, alb, metal);
-                    const f3 H = normalize(add(V, L));
-                    const float NDF = distribution_ggx(N, H, rough);
-                    const float G = geometry_smith(N, V, L, rough);
-                    const f3 F = fresnel_schlick(gmax(dot(H, V), 0.0f), F0);
-                    f3 kD = ssub(1.0f, F);
-                    kD = muls(kD, 1.0f - metal);
-                    const f3 numerator = muls(F, NDF * G);
-                    const float denominator = 4.0f * gmax(dot(N, V), 0.0f) * gmax(dot(N, L), 0.0f);
-                    const f3 specular = divs(numerator, gmax(denominator, 0.001f));
-                    const float NdotL = gmax(dot(N, L), 0.0f);
-                    const f3 cnee = muls(mul(add(divs(mul(kD, alb), kPi), specular), att), NdotL);
-                    ++rays_px;
-                    ++c_rays;
-                    ++c_shadow;
-                    ++ei;
-                    if (!(cnee.x == 0.0f && cnee.y == 0.0f && cnee.z == 0.0f)) {
-                        ++c_traced;
-                        if constexpr (POOL) {
-                            float4* sl = reinterpret_cast<float4*>(rec + kRecSlots + nray * 32u);
-                            sl[0] = make_float4(L.x, L.y, L.z, tl);
-                            sl[1] = make_float4(cnee.x, cnee.y, cnee.z, 0.0f);
-                            if (++nray >= kRaySlots) break;
-                        } else {
+                    salt = sv;
+                    rays_px += w;
+                    c_rays += w;
+                    c_shadow += w;
+                    ei += w;
+                    nitem = w;
+                    *reinterpret_cast<float4*>(rec + kRecHead) =
+                        make_float4(P.x, P.y, P.z, __uint_as_float(BATCH ? (job << 1) | 1u : 1u));
+                } else {
+                    while (ei < j.sc.n_emitters) {
+                        const float s0 = salt;
+                        salt = salt + 1.0f;      // the sample's two draws (:464-465)
+                        salt = salt + 1.0f;
+                        ++rays_px;
+                        ++c_rays;
+                        ++c_shadow;
+                        if (nee_sample(j, h, ei++, s0, uvx, uvy, d0, t0max, c0)) {
+                            ++c_traced;
                             o0 = P;
-                            d0 = L;
-                            t0max = tl;
-                            c0 = cnee;
                             nray = 1;
                             break;
                         }
                     }
                 }
-                if constexpr (POOL)
-                    *reinterpret_cast<float4*>(rec + kRecHead) =
-                        make_float4(P.x, P.y, P.z, __uint_as_float(BATCH ? (job << 1) | 1u : 1u));
+            }
+        }
+
+        if constexpr (POOL) {
+            // Path state that the wave's evaluation and tracing do not need
+            // waits in the record, so it holds no registers meanwhile.
+            float4* park = reinterpret_cast<float4*>(rec);
+            park[0] = make_float4(ro.x, ro.y, ro.z, dj.x);
+            park[1] = make_float4(rd.x, rd.y, rd.z, dj.y);
+            park[2] = make_float4(P.x, P.y, P.z, dj.z);
+            park[3] = make_float4(N.x, N.y, N.z, 0.0f);
+            // The wave evaluates every published (lane, emitter) pair: lane l's
+            // pairs are items s_pre[l] .. s_pre[l + 1] - 1.  A nonzero
+            // contribution goes to slot k of the owner's record (emitter
+            // ei0 + k) and sets bit k of its mask.
+            const unsigned long long lt = (1ull << __lane_id()) - 1ull;
+            uint32_t pre = 0, tot = 0;
+            for (uint32_t b = 0; (1u << b) <= kRaySlots; ++b) {   // the bits of nitem (<= kRaySlots)
+                const unsigned long long m = __ballot((nitem >> b) & 1u);
+                pre += (uint32_t)__popcll(m & lt) << b;
+                tot += (uint32_t)__popcll(m) << b;
+            }
+            s_pre[threadIdx.x] = pre;
+            s_msk[threadIdx.x] = 0u;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (uint32_t base = 0; base < tot; base += 64u) {
+                const uint32_t e = base + __lane_id();
+                if (e < tot) {
+                    uint32_t lo = 0, plo = 0;
+                    for (uint32_t step = 32; step > 0; step >>= 1) {
+                        const uint32_t p = s_pre[wbase + lo + step];
+                        if (p <= e) { lo += step; plo = p; }
+                    }
+                    const uint32_t k = e - plo;
+                    const char* orec = wrec + lo * kLaneBytes;
+                    const float4 hd = *reinterpret_cast<const float4*>(orec + kRecHead);
+                    const float4 h0 = reinterpret_cast<const float4*>(orec + kRecHit)[0];
+                    const float4 h1 = reinterpret_cast<const float4*>(orec + kRecHit)[1];
+                    const float4 h2 = reinterpret_cast<const float4*>(orec + kRecHit)[2];
+                    const float sk = reinterpret_cast<const float*>(orec + kRecSalt)[k];
+                    const JobDesc& oj = BATCH ? A.jobs[__float_as_uint(hd.w) >> 1] : A.job;
+                    NeeHit h;
+                    h.P = f3{hd.x, hd.y, hd.z};
+                    h.N = f3{h0.x, h0.y, h0.z};
+                    h.V = normalize(sub(ld3(oj.cam_pos), h.P));
+                    h.alb = f3{h1.x, h1.y, h1.z};
+                    h.rough = h1.w;
+                    h.metal = h2.x;
+                    h.flags = __float_as_uint(h2.w);
+                    f3 L, c;
+                    float tl;
+                    if (nee_sample(oj, h, __float_as_uint(h2.z) + k, sk, h0.w, h2.y, L, tl, c)) {
+                        float4* sl = reinterpret_cast<float4*>(const_cast<char*>(orec) + kRecSlots + k * 32u);
+                        sl[0] = make_float4(L.x, L.y, L.z, tl);
+                        sl[1] = make_float4(c.x, c.y, c.z, 0.0f);
+                        atomicOr(s_msk + wbase + lo, 1u << k);
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (has && phase != kPhTrace) {
+                nray = (uint32_t)__popc(s_msk[threadIdx.x]);
+                c_traced += nray;
+            } else if (nray) {
+                s_msk[threadIdx.x] = 1u;   // the path segment is slot 0
             }
         }
 
@@ -340,14 +433,8 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
             if (phase == kPhTrace) tr = t;
             else occ = t.hit ? 1u : 0u;
         } else {
-            // The wave traces all queued rays together from its pool.  Path
-            // state that tracing does not need waits in the record, so it
-            // holds no registers meanwhile.
-            float4* park = reinterpret_cast<float4*>(rec);
-            park[0] = make_float4(ro.x, ro.y, ro.z, dj.x);
-            park[1] = make_float4(rd.x, rd.y, rd.z, dj.y);
-            park[2] = make_float4(P.x, P.y, P.z, dj.z);
-            park[3] = make_float4(N.x, N.y, N.z, 0.0f);
+            // The wave traces all queued rays together from its pool.
+            const float4* park = reinterpret_cast<const float4*>(rec);
             const unsigned long long lt = (1ull << __lane_id()) - 1ull;
             uint32_t pre = 0, tot = 0;
             for (uint32_t b = 0; (1u << b) <= kRaySlots; ++b) {   // the bits of nray (<= kRaySlots)
@@ -362,8 +449,8 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            traverse_pool<BLOCK, COUNT, BATCH>(BATCH ? A.jobs : &A.job, tot, s_pre + wbase, s_occ + wbase,
-                                               s_res + wbase, wrec, stk, wc);
+            traverse_pool<BLOCK, COUNT, BATCH>(BATCH ? A.jobs : &A.job, tot, s_pre + wbase, s_msk + wbase,
+                                               s_occ + wbase, s_res + wbase, wrec, stk, wc);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -438,22 +525,23 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
                         const f3 alb = ld3(m->albedo);
                         const float nn = dot(N, N);
                         const float rough = m->roughness, metal = m->metalness;
-                        nee_zero = j.sc.emitters_bounded && metal == 1.0f && rough == 0.0f &&
+                        const bool nee_zero = j.sc.emitters_bounded && metal == 1.0f && rough == 0.0f &&
                                    fabsf(alb.x) < 1e30f && fabsf(alb.y) < 1e30f && fabsf(alb.z) < 1e30f &&
                                    nn < 0.9999f;
                         const float rp = rough + 1.0f;
                         const float k = (rp * rp) / 8.0f;
                         const float a = rough * rough;
                         const float a2 = a * a;
-                        nee_fin = j.sc.emitters_bounded && fabsf(alb.x) <= 16.0f && fabsf(alb.y) <= 16.0f &&
+                        const bool nee_fin = j.sc.emitters_bounded && fabsf(alb.x) <= 16.0f && fabsf(alb.y) <= 16.0f &&
                                   fabsf(alb.z) <= 16.0f && fabsf(metal) <= 16.0f && k >= 0x1p-10f && k <= 1.0f &&
                                   nn <= 1024.0f &&
                                   (a2 >= 1.0f || 1.0f - (nn * (1.0f + 0x1p-16f)) * (1.0f - a2) >= 0x1p-10f);
                         // Other mirror hits: kD = 0 and NDF = 0 / (pi (1 - NdotH^2)^2)
                         // is 0 unless NdotH^2 rounds to exactly 1 (then NaN).
-                        nee_mirror = !nee_zero && j.sc.emitters_bounded && metal == 1.0f && rough == 0.0f &&
-                                     fabsf(alb.x) <= 16.0f && fabsf(alb.y) <= 16.0f && fabsf(alb.z) <= 16.0f &&
-                                     nn <= 1024.0f;
+                        const bool nee_mirror = !nee_zero && j.sc.emitters_bounded && metal == 1.0f &&
+                                                rough == 0.0f && fabsf(alb.x) <= 16.0f && fabsf(alb.y) <= 16.0f &&
+                                                fabsf(alb.z) <= 16.0f && nn <= 1024.0f;
+                        nee_flags = (nee_zero ? kNeeZero : 0u) | (nee_fin ? kNeeFin : 0u) | (nee_mirror ? kNeeMirror : 0u);
                     }
                     if (j.sc.n_emitters > 0) phase = kPhNee;
                     else fin_diffuse = true;
@@ -480,14 +568,16 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
                 }
             }
         } else {
-            for (uint32_t k = 0; k < nray; ++k) {
-                if ((occ >> k) & 1u) continue;                       // :473
-                if constexpr (POOL) {
+            if constexpr (POOL) {
+                uint32_t bits = s_msk[threadIdx.x] & ~occ;           // unoccluded (:473), in emitter order
+                while (bits) {
+                    const uint32_t k = (uint32_t)__builtin_ctz(bits);
+                    bits &= bits - 1u;
                     const float4 c = reinterpret_cast<const float4*>(rec + kRecSlots + k * 32u)[1];
                     e = add(e, f3{c.x, c.y, c.z});                   // :502
-                } else {
-                    e = add(e, c0);
                 }
+            } else if (nray && !(occ & 1u)) {
+                e = add(e, c0);                                      // :473, :502
             }
             if (ei >= j.sc.n_emitters) fin_diffuse = true;
         }

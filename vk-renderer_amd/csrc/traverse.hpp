@@ -313,15 +313,15 @@ __device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, floa
 constexpr uint32_t kNone = 0xFFFFu;
 
 // Traces every ray the lanes of a wave queued for this step.  Lanes take rays
-// 0..total of the wave (lane l queued rays pre[l] .. pre[l+1]-1) until none
-// is left, so a step costs the wave's total work plus the last
+// 0..total of the wave (lane l queued rays pre[l] .. pre[l+1]-1, held in the
+// slots of the set bits of msk[l]) until none is left, so a step costs the wave's total work plus the last
 // rays' tail instead of its longest lane.  A shadow ray sets bit `slot` of
 // occ[owner] when occluded; the closest-hit ray of lane `owner` writes
 // (prim, t) to res[owner].  `wrec` is the record of the wave's lane 0.
 template <int BLOCK, bool COUNT, bool BATCH>
 __device__ __forceinline__ void traverse_pool(const JobDesc* jobs, uint32_t total, const uint32_t* pre,
-                                              uint32_t* occ, uint2* res, const char* __restrict__ wrec,
-                                              uint32_t* stk, WorkCnt& wc)
+                                              const uint32_t* msk, uint32_t* occ, uint2* res,
+                                              const char* __restrict__ wrec, uint32_t* stk, WorkCnt& wc)
 {
     // The job (scene, epsilon) of the ray in flight: a batched wave can hold
     // pixels of several jobs, and a lane may trace another lane's ray.
@@ -355,7 +355,17 @@ __device__ __forceinline__ void traverse_pool(const JobDesc* jobs, uint32_t tota
                         const uint32_t p = pre[lo + step];
                         if (p <= e) { lo += step; plo = p; }
                     }
-                    cur = (lo << 5) | (e - plo);
+                    // Its (e - plo)-th ray is in the slot of that set bit of msk[lo].
+                    uint32_t m = msk[lo], rem = e - plo, slot = 0;
+                    for (uint32_t wdt = 16; wdt > 0; wdt >>= 1) {
+                        const uint32_t c = (uint32_t)__popc(m & ((1u << wdt) - 1u));
+                        if (rem >= c) {
+                            rem -= c;
+                            m >>= wdt;
+                            slot += wdt;
+                        }
+                    }
+                    cur = (lo << 5) | slot;
                     const char* rec = wrec + (cur >> 5) * kLaneBytes;
                     const float4 h = *reinterpret_cast<const float4*>(rec + kRecHead);
                     const float4 s = *reinterpret_cast<const float4*>(rec + kRecSlots + (cur & 31u) * 32u);
