@@ -63,7 +63,7 @@ struct DevCounters {
 constexpr uint32_t kDefaultChunk = 8u;
 
 // One-pixel-per-lane launches cover at most this many lanes each, which
-// bounds their ray records (2^21 x kLaneBytes = 2.4 GB).
+// bounds their ray records (2^21 x kLaneBytes = 2.7 GB).
 constexpr uint32_t kMaxPlainLanes = 1u << 21;
 
 // Per-lane ray record of the path kernel (one per lane of a launch, in
