@@ -294,6 +294,22 @@ def test_ray_pool_variants_identical(ctx, torch_cuda):
     assert st["prim_id"] == 1.0 and st["radiance_bits"] == 1.0, st
 
 
+def test_emitter_windows_full_frame(ctx, torch_cuda):
+    """The pooled kernel evaluates next-event samples in windows of 32
+    emitters; 69, 33 and 32 emitters give three, two and one window per
+    diffuse hit.  Whole frames against the oracle, every bit."""
+    for n, seed, n_em in ((140, 0, 69), (66, 3, 33), (60, 5, 32)):
+        wl = scenes.many_emitters(n, seed)
+        sid = ctx.upload(wl.scene)
+        assert ctx.info(sid)["n_emitters"] == n_em
+        g = gpu_trace(ctx, sid, wl)
+        r = oracle_ffi.trace(wl.scene, wl.frame, wl.width, wl.height, wl.spp, wl.max_depth)
+        st = compare(g, r)
+        _report(wl.name, st)
+        assert st["prim_id"] == 1.0 and st["radiance_bits"] == 1.0 and st["ray_count"] == 1.0, st
+        assert st["rgba8"] == 1.0, st
+
+
 def test_plain_launch_split(ctx, torch_cuda):
     """One pixel per lane above 2^21 pixels runs as several launches (bounded
     ray records); the image is identical to the persistent schedule's."""

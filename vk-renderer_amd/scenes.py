@@ -189,6 +189,22 @@ def soup100k(seed: int = 0, n_tris: int = 100_000, width: int = 1920, height: in
                     width, height, spp, max_depth)
 
 
+def many_emitters(n_spheres: int = 140, seed: int = 0, width: int = 64, height: int = 48, spp: int = 2,
+                  max_depth: int = 3) -> Workload:
+    """Spheres of which about half emit (diffuse / metal / emissive /
+    emissive), a 400-triangle soup and a ground plane: with more than 32
+    emitters a diffuse hit's next-event samples span several evaluation
+    windows of the pooled kernel (seed 0: 69 emitters; n_spheres 66, seed 3:
+    33; n_spheres 60, seed 5: 32)."""
+    rng = rng_for(11, seed)
+    sph = random_spheres(rng, n_spheres, -12.0, 12.0, kinds=("diffuse", "metal", "emissive", "emissive"))
+    tris = random_soup(rng, 400, lo=-15.0, hi=15.0)
+    cam = vkr.camera_from_angles((0.0, 0.0, -40.0), 0.0, 90.0)
+    sc = vkr.Scene(tris=tris, spheres=sph, planes=ground(-13.0))
+    return Workload(f"many_emitters_{n_spheres}_{seed}", sc, frame(cam, np.float32(width) / np.float32(height)),
+                    width, height, spp, max_depth)
+
+
 def far_sphere(res: int = 128, dist: float = 2000.0, span: float = 1.5e-3) -> Workload:
     """A unit sphere seen from `dist` away through a narrow camera (the image
     spans +-span*dist around it), plus a mirror-triangle soup and two more
