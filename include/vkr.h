@@ -196,6 +196,23 @@ vkr_error vkr_scene_get_info(vkr_ctx* ctx, uint32_t scene, vkr_scene_info* out);
  * Build extension: the reference has no parameters (Tracer.comp:186-202). */
 vkr_error vkr_scene_set_sphere_params(vkr_ctx* ctx, uint32_t scene, const float* params, uint32_t n);
 
+/* Scene files (build extension; the reference has no scene file, only the
+ * bytes it copies into its SceneData SSBO, Source/GraphicsDevice.cpp:796-822).
+ * A file holds the arrays of a vkr_scene_desc in their std430 layouts behind
+ * a 64-byte little-endian header:
+ *   char magic[8] = "VKRSCENE"; u32 version = 1; u32 flags;
+ *   u32 n_tris, n_spheres, n_planes, n_materials; u32 has_tri_material;
+ *   u32 max_leaf_size, max_depth; f32 traversal_cost; u32 reserved[4] = 0
+ * then tri_material (48 B, when has_tri_material), tris (48 B each),
+ * spheres (64 B each), planes (64 B each), materials (48 B each).
+ * Host only: these calls touch no device and need no context. */
+vkr_error vkr_scene_save(const char* path, const vkr_scene_desc* desc);
+/* Reads a scene file into *out (arrays allocated by the library); a file
+ * whose size does not match its header is rejected. */
+vkr_error vkr_scene_load(const char* path, vkr_scene_desc* out);
+/* Releases the arrays of a vkr_scene_load result and zeroes *desc. */
+void vkr_scene_free(vkr_scene_desc* desc);
+
 /* ------------------------------------------------------------------------ */
 /* Tracing                                                                   */
 /* ------------------------------------------------------------------------ */

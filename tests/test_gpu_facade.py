@@ -80,3 +80,27 @@ def test_pose_replay_matches_oracle(tmp_path):
         fr["seed"] = np.float32(seed)
         ref = oracle_ffi.trace(wl.scene, fr, res, res, 4, 4)["rgba8"]
         assert np.array_equal(got, ref), slot
+
+
+@pytest.mark.gpu
+def test_scene_file_matches_oracle(tmp_path):
+    """--scene traces a scene file (vkr_scene_save) in place of the reference's
+    constant scene; the image matches the oracle on the same arrays."""
+    import torch
+    import vkr
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    res = 96
+    wl = scenes.soup100k(seed=9, n_tris=2000, width=res, height=res, n_spheres=24)
+    path = str(tmp_path / "soup.vkrs")
+    wl.scene.save(path)
+    prefix = str(tmp_path / "s")
+    r = _run(["--frames", "1", "--res", str(res), "--seeds", "0.5", "--scene", path, "--raw", prefix])
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.fromfile(prefix + ".traced0.rgba", np.uint8).reshape(-1, 4)
+    sc = vkr.Scene.load(path)
+    fr = scenes.reference_scene(res, res).frame.copy()       # camera of Main.cpp:134-141
+    fr["aspect_ratio"] = np.float32(1024) / np.float32(768)
+    fr["seed"] = np.float32(0.5)
+    ref = oracle_ffi.trace(sc, fr, res, res, 4, 4)["rgba8"]
+    assert np.array_equal(got, ref)

@@ -3,10 +3,12 @@
 // print the frame rate, and write the presented image.
 //
 //   vkr_headless [--frames N] [--res R] [--present WxH] [--seeds a,b,c]
-//                [--poses file] [--out image.ppm] [--raw prefix]
+//                [--poses file] [--scene file.vkrs] [--out image.ppm] [--raw prefix]
 // --poses replays recorded camera input in place of Main.cpp:165-190's
 // keyboard/mouse handling: one "x y z pitch yaw" line per frame (cycled;
 // '#' starts a comment), each applied through Camera::update.
+// --scene traces a scene file (vkr_scene_save) instead of the reference's
+// constant scene.
 // --raw writes prefix.traced0.rgba, prefix.traced1.rgba, prefix.present.rgba
 // and prefix.seeds.txt for checking against the CPU restatement.
 #include <chrono>
@@ -53,7 +55,7 @@ int main(int argc, char** argv)
     int frames = 60;
     unsigned res = 1024, pw = 1024, ph = 768;
     std::vector<float> seeds;
-    std::string out, raw, poses_path;
+    std::string out, raw, poses_path, scene_path;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         auto next = [&]() { return i + 1 < argc ? std::string(argv[++i]) : std::string(); };
@@ -63,6 +65,7 @@ int main(int argc, char** argv)
         else if (a == "--out") out = next();
         else if (a == "--raw") raw = next();
         else if (a == "--poses") poses_path = next();
+        else if (a == "--scene") scene_path = next();
         else if (a == "--seeds") {
             const std::string s = next();
             size_t p = 0;
@@ -94,6 +97,20 @@ int main(int argc, char** argv)
         return 1;
     }
     device.SetPresentExtent(pw, ph);
+    if (!scene_path.empty()) {
+        // A scene file (vkr_scene_save) in place of the reference's constant scene.
+        vkr_scene_desc desc{};
+        if (vkr_scene_load(scene_path.c_str(), &desc) != VKR_SUCCESS) {
+            std::printf("{\"error\": %d, \"message\": \"%s\"}\n", 4, vkr_last_error_string());
+            return 2;
+        }
+        const auto e = device.SetScene(desc);
+        vkr_scene_free(&desc);
+        if (e != GraphicsDevice::Error::SUCCESS) {
+            std::printf("{\"error\": %d, \"message\": \"%s\"}\n", (int)e, vkr_last_error_string());
+            return 1;
+        }
+    }
     if (!seeds.empty()) device.SetSeeds(seeds);
 
     Camera camera;                                                                       // Main.cpp:134-139

@@ -165,6 +165,9 @@ def load(path: str = LIB_PATH):
         "vkr_camera_from_angles": (e, [C.POINTER(C.c_float), C.c_float, C.c_float, C.POINTER(CameraC)]),
         "vkr_debug_eval": (e, [vp, u32, vp, vp, u32, vp]),
         "vkr_autotune": (e, [vp, u32, vp, C.POINTER(TraceParams), C.POINTER(Outputs), u32, C.POINTER(TuneResult)]),
+        "vkr_scene_save": (e, [C.c_char_p, C.POINTER(SceneDesc)]),
+        "vkr_scene_load": (e, [C.c_char_p, C.POINTER(SceneDesc)]),
+        "vkr_scene_free": (None, [C.POINTER(SceneDesc)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -204,6 +207,35 @@ class Scene:
     tri_material: np.ndarray = field(default_factory=lambda: MIRROR.copy())
     materials: np.ndarray = field(default_factory=lambda: np.zeros(0, MATERIAL))
     flags: int = 0
+
+    def save(self, path: str, max_leaf: int = 0, max_depth: int = 0, traversal_cost: float = 0.0):
+        """Writes the scene file of vkr_scene_save (host only, no device)."""
+        tris = np.ascontiguousarray(self.tris, TRIANGLE)
+        sph = np.ascontiguousarray(self.spheres, SPHERE)
+        pln = np.ascontiguousarray(self.planes, PLANE)
+        tm = np.ascontiguousarray(np.asarray(self.tri_material, MATERIAL).reshape(1))
+        mats = np.ascontiguousarray(self.materials, MATERIAL)
+        d = SceneDesc(_ptr(tris), len(tris), _ptr(sph), len(sph), _ptr(pln), len(pln), _ptr(tm),
+                      _ptr(mats), len(mats), self.flags, max_leaf, max_depth, traversal_cost)
+        check(load().vkr_scene_save(os.fsencode(path), C.byref(d)), "vkr_scene_save")
+
+    @staticmethod
+    def load(path: str) -> "Scene":
+        """Reads a scene file (vkr_scene_load) into host arrays."""
+        lib = load()
+        d = SceneDesc()
+        check(lib.vkr_scene_load(os.fsencode(path), C.byref(d)), "vkr_scene_load")
+        try:
+            def arr(ptr, n, dt):
+                if not n:
+                    return np.zeros(0, dt)
+                return np.frombuffer(C.string_at(ptr, n * dt.itemsize), dt).copy()
+            tm = arr(d.tri_material, 1, MATERIAL)[0] if d.tri_material else MIRROR.copy()
+            return Scene(tris=arr(d.tris, d.n_tris, TRIANGLE), spheres=arr(d.spheres, d.n_spheres, SPHERE),
+                         planes=arr(d.planes, d.n_planes, PLANE), tri_material=tm,
+                         materials=arr(d.materials, d.n_materials, MATERIAL), flags=d.flags)
+        finally:
+            lib.vkr_scene_free(C.byref(d))
 
 
 def _ptr(a: np.ndarray | None):
