@@ -290,6 +290,12 @@ vkr_error vkr_trace_batch(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n_jo
 vkr_error vkr_batch_graph_create(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n_jobs,
                                  uint32_t block, uint32_t chunk, void* stream, uint32_t* out_graph);
 vkr_error vkr_graph_launch(vkr_ctx* ctx, uint32_t graph, void* stream);
+/* Replaces the per-frame data (camera, aspect_ratio, seed, light_pos) of job
+ * `job` of a captured graph, in order on `stream`, so that a graph replays
+ * Draw's push constants (Source/GraphicsDevice.cpp:1258-1262) frame after
+ * frame; the next vkr_graph_launch on that stream traces with them. */
+vkr_error vkr_graph_set_frame(vkr_ctx* ctx, uint32_t graph, uint32_t job, const vkr_frame_data* frame_data,
+                              void* stream);
 vkr_error vkr_graph_destroy(vkr_ctx* ctx, uint32_t graph);
 
 /* Build extension: the project autotuner.  Times the path kernel on this

@@ -143,3 +143,29 @@ def test_autotune_picks_a_valid_shape(ctx, torch_cuda):
     torch.cuda.synchronize()
     for k in KEYS:
         assert torch.equal(a[k], p[k]), k
+
+
+def test_graph_set_frame_replays_new_push_constants(ctx, torch_cuda):
+    """A captured launch replayed with new per-frame data (camera, aspect,
+    seed) traces exactly what vkr_trace_frame traces with that data."""
+    torch = torch_cuda
+    wl = scenes.spheres64()
+    sid = ctx.upload(wl.scene)
+    b = _bufs(torch, wl)
+    gid = ctx.batch_graph([ctx.job(sid, wl.frame, wl.params(), **b)])
+    try:
+        for k, (pos, pitch, yaw, seed) in enumerate((((0.0, 0.0, -30.0), 0.0, 90.0, 0.25),
+                                                     ((4.0, 3.0, -28.0), -8.0, 97.0, 0.75))):
+            fr = wl.frame.copy()
+            fr["camera"] = vkr.camera_from_angles(pos, pitch, yaw)
+            fr["seed"] = np.float32(seed)
+            fr["aspect_ratio"] = np.float32(1.25 + k)
+            ctx.graph_set_frame(gid, 0, fr)
+            ctx.graph_launch(gid)
+            ref = _bufs(torch, wl)
+            ctx.trace(sid, fr, wl.params(), **ref)
+            torch.cuda.synchronize()
+            for key in KEYS:
+                assert torch.equal(b[key], ref[key]), (k, key)
+    finally:
+        ctx.graph_destroy(gid)

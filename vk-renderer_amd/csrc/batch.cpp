@@ -159,6 +159,7 @@ vkr_error vkr_batch_graph_create(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32
     VKR_HIP(g->pool.alloc(64));
     VKR_HIP(g->counters.alloc(sizeof(DevCounters)));
     g->total = plan.total;
+    g->n_jobs = n_jobs;
 
     // Capture on a private stream: a fill kernel resets the pool cursor and the
     // counters (no memset nodes), then the persistent batch kernel.
@@ -211,6 +212,18 @@ vkr_error vkr_graph_launch(vkr_ctx* ctx, uint32_t graph, void* stream)
     VKR_HIP(hipEventRecord(ctx->ev1, st));
     ctx->timed = true;
     ctx->last_counters = g.counters.as<DevCounters>();
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_graph_set_frame(vkr_ctx* ctx, uint32_t graph, uint32_t job, const vkr_frame_data* frame_data,
+                              void* stream)
+{
+    if (!ctx || !frame_data || graph >= ctx->graphs.size() || !ctx->graphs[graph])
+        return fail(VKR_INVALID_ARGUMENT, "unknown graph or NULL frame data");
+    Graph& g = *ctx->graphs[graph];
+    if (job >= g.n_jobs) return fail(VKR_INVALID_ARGUMENT, "job index out of range");
+    VKR_HIP(hipSetDevice(ctx->device));
+    VKR_HIP(launch_set_frame(g.jobs.as<JobDesc>() + job, *frame_data, static_cast<hipStream_t>(stream)));
     return VKR_SUCCESS;
 }
 

@@ -85,6 +85,7 @@ struct Graph {
     hipGraphExec_t exec = nullptr;
     DevBuf jobs, job_start, pool, counters, lanes;
     uint64_t total = 0;
+    uint32_t n_jobs = 0;
     ~Graph()
     {
         if (exec) (void)hipGraphExecDestroy(exec);

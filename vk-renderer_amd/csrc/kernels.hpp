@@ -127,6 +127,9 @@ hipError_t launch_filter(const uint8_t* cur, const uint8_t* prev, uint32_t sw, u
 hipError_t launch_adjoint(const AdjointLaunch& a, uint32_t stack, hipStream_t s);
 hipError_t launch_debug(uint32_t op, const float* in, float* out, uint32_t n, hipStream_t s);
 hipError_t launch_zero_counters(DevCounters* c, uint32_t* pool, hipStream_t s);
+// Writes the per-frame fields of a job in a device job table (camera,
+// aspect, seed, light position), in stream order.
+hipError_t launch_set_frame(JobDesc* job, const vkr_frame_data& fd, hipStream_t s);
 // Occupancy-derived persistent grid size for (block, stack).
 uint32_t persistent_grid(const PathVariant& v, int device);
 // Largest stack depth with a compiled kernel variant (0 when none fits).

@@ -1009,6 +1009,26 @@ hipError_t launch_debug(uint32_t op, const float* in, float* out, uint32_t n, hi
     return hipGetLastError();
 }
 
+__global__ void k_set_frame(JobDesc* job, vkr_frame_data fd)
+{
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < 3; ++k) {
+        job->cam_pos[k] = fd.camera.pos[k];
+        job->cam_dir[k] = fd.camera.dir[k];
+        job->cam_right[k] = fd.camera.right[k];
+        job->cam_up[k] = fd.camera.up[k];
+        job->light_pos[k] = fd.light_pos[k];
+    }
+    job->aspect = fd.aspect_ratio;
+    job->seed = fd.seed;
+}
+
+hipError_t launch_set_frame(JobDesc* job, const vkr_frame_data& fd, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_frame, dim3(1), dim3(64), 0, s, job, fd);
+    return hipGetLastError();
+}
+
 hipError_t launch_zero_counters(DevCounters* c, uint32_t* pool, hipStream_t s)
 {
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, c, pool);

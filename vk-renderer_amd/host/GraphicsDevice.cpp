@@ -118,6 +118,10 @@ struct GraphicsDevice::State {
     size_t seed_pos = 0;
     float last_seed = 0.0f;
     vkr_error last_error = VKR_SUCCESS;
+    // One captured launch per frame slot, replayed every frame with that
+    // frame's push constants (created on the slot's first Draw).
+    std::vector<uint32_t> graphs;
+    static constexpr uint32_t kNoGraph = 0xFFFFFFFFu;
 };
 
 GraphicsDevice::Error GraphicsDevice::Construct(const CreateInfo& info)
@@ -145,6 +149,7 @@ GraphicsDevice::Error GraphicsDevice::Construct(const CreateInfo& info)
         if (ev) s->fences.push_back(ev);
     }
     ok = ok && hipMalloc(&s->present, (size_t)s->present_w * s->present_h * 4u) == hipSuccess;
+    s->graphs.assign(s->frames, State::kNoGraph);
     state = s;
     if (!ok) {
         Destruct();
@@ -212,7 +217,15 @@ void GraphicsDevice::Draw(const FrameData& frame_data)
             tp.sched = VKR_SCHED_AUTO;
             vkr_outputs out{};
             out.rgba8 = s->traced[slot];
-            s->last_error = vkr_trace_frame(s->ctx, s->scene, &fd, &tp, &out, s->stream);
+            if (s->graphs[slot] == State::kNoGraph) {
+                const vkr_scene_job job{s->scene, fd, tp, out};
+                s->last_error = vkr_batch_graph_create(s->ctx, &job, 1, 0, 0, s->stream, &s->graphs[slot]);
+                if (s->last_error != VKR_SUCCESS) s->graphs[slot] = State::kNoGraph;
+            }
+            if (s->graphs[slot] != State::kNoGraph) {
+                s->last_error = vkr_graph_set_frame(s->ctx, s->graphs[slot], 0, &fd, s->stream);
+                if (s->last_error == VKR_SUCCESS) s->last_error = vkr_graph_launch(s->ctx, s->graphs[slot], s->stream);
+            }
         } else {
             // Both bindings are fixed to traced images 0 and 1 (:964-983).
             const uint8_t* cur = s->traced[0];
@@ -257,6 +270,12 @@ GraphicsDevice::Error GraphicsDevice::SetScene(const vkr_scene_desc& desc)
     const vkr_error e = vkr_scene_upload(state->ctx, &desc, &id);
     if (e != VKR_SUCCESS) return map_error(e);
     WaitIdle();
+    // The captured launches hold the old scene's device data: drop them
+    // before the scene goes (the next Draw of each slot captures anew).
+    for (uint32_t& g : state->graphs) {
+        if (g != State::kNoGraph) (void)vkr_graph_destroy(state->ctx, g);
+        g = State::kNoGraph;
+    }
     (void)vkr_scene_destroy(state->ctx, state->scene);
     state->scene = id;
     return Error::SUCCESS;

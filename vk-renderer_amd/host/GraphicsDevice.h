@@ -2,7 +2,9 @@
 // kept in shape on top of the vkr C ABI.  Construct / Draw / WaitIdle /
 // Destruct behave like the reference's: Draw traces the frame-in-flight's
 // image with the path kernel (Tracer.comp semantics, 4 spp, 4 bounces) and
-// runs the present filter into the backbuffer.  The machine is headless, so
+// runs the present filter into the backbuffer.  Each frame slot's trace is
+// a captured hipGraph, replayed every Draw with that frame's push constants
+// (vkr_graph_set_frame).  The machine is headless, so
 // the "swapchain" is a device image of the present extent (1024x768 unless
 // set, the window size of Source/Main.cpp:92).
 #pragma once

@@ -156,6 +156,7 @@ def load(path: str = LIB_PATH):
         "vkr_trace_batch": (e, [vp, C.POINTER(SceneJob), u32, u32, u32, vp]),
         "vkr_batch_graph_create": (e, [vp, C.POINTER(SceneJob), u32, u32, u32, vp, C.POINTER(u32)]),
         "vkr_graph_launch": (e, [vp, u32, vp]),
+        "vkr_graph_set_frame": (e, [vp, u32, u32, vp, vp]),
         "vkr_graph_destroy": (e, [vp, u32]),
         "vkr_adjoint": (e, [vp, u32, vp, C.POINTER(TraceParams), vp, vp, vp]),
         "vkr_last_counters": (e, [vp, C.POINTER(Counters)]),
@@ -368,6 +369,12 @@ class Context:
     def graph_launch(self, gid: int, stream=None):
         st = _stream() if stream is None else stream
         check(self.lib.vkr_graph_launch(self.h, gid, st), "vkr_graph_launch")
+
+    def graph_set_frame(self, gid: int, job: int, frame: np.ndarray, stream=None):
+        """Replaces job `job`'s per-frame data (camera, aspect, seed, light) in a captured graph."""
+        fr = np.ascontiguousarray(np.asarray(frame, FRAME).reshape(1))
+        st = _stream() if stream is None else stream
+        check(self.lib.vkr_graph_set_frame(self.h, gid, job, fr.ctypes.data, st), "vkr_graph_set_frame")
 
     def graph_destroy(self, gid: int):
         check(self.lib.vkr_graph_destroy(self.h, gid), "vkr_graph_destroy")
