@@ -1,0 +1,237 @@
+// bvh_accel.cpp - the CPU baseline's variant (ii): the oracle's path program
+// tracing the same BVH as the GPU (the build's own host builder,
+// vk-renderer_amd/csrc/bvh.cpp, with the same defaults as vkr_scene_upload).
+//
+// TEST INFRASTRUCTURE ONLY: a baseline for the benchmark's CPU leg and a
+// cross-check in the test suite; the product never links it.
+//
+// trace_ray (Assets/Tracer.comp:374-431) accepts `t > EPSILON && t < best +
+// EPSILON` over triangles then spheres in scan order, then planes with
+// `t < best - EPSILON`.  The smallest hit t* is always accepted by the scan,
+// so only hits below t* + 4 EPSILON can still change the winner: they are all
+// kept (no cap) and replayed in scan order after the traversal.  A winner
+// reached through a longer chain of hits each within EPSILON of the previous
+// one is a near-tie the same way as on the GPU.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../vk-renderer_amd/csrc/bvh.hpp"
+#include "vkr_oracle.h"
+
+namespace {
+
+struct Accel {
+    vkr::BuildOutput bvh;
+    std::vector<otri> tris;   // scan order, as given
+    std::vector<osph> sph;
+    std::vector<opln> pln;
+};
+
+struct F3 { float x, y, z; };
+inline F3 sub3(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline F3 cross3(F3 a, F3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+// The reference's intersection routines (Assets/Tracer.comp:314-372), as the
+// oracle restates them (vkr_oracle.c: sphere_t, plane_t, tri_t).
+float sphere_t(F3 o, F3 d, F3 c, float r)
+{
+    const F3 oc = sub3(o, c);
+    const float b = 2.0f * dot3(oc, d);
+    const float cc = dot3(oc, oc) - r * r;
+    const float h = b * b - 4.0f * cc;
+    if (h < 0.0f) return -1.0f;
+    return (-b - std::sqrt(h)) / 2.0f;
+}
+
+float plane_t(F3 o, F3 d, F3 n, float len)
+{
+    const float dd = dot3(d, n);
+    const float dist = -(len + dot3(o, n)) / dd;
+    const float sg = (dd - 0.0f) > 0.0f ? 1.0f : ((dd - 0.0f) < 0.0f ? -1.0f : 0.0f);
+    return std::fabs(sg) * std::fmax(dist, 0.0f);
+}
+
+float tri_t(F3 o, F3 d, F3 v0, F3 v1, F3 v2, float eps)
+{
+    const F3 e1 = sub3(v1, v0), e2 = sub3(v2, v0);
+    const F3 pvec = cross3(d, e2);
+    const float det = dot3(e1, pvec);
+    if (det < eps) return -1.0f;
+    const float inv = 1.0f / det;
+    const F3 tvec = sub3(o, v0);
+    const float u = dot3(tvec, pvec) * inv;
+    if (u < 0.0f || u > 1.0f) return -1.0f;
+    const F3 qvec = cross3(tvec, e1);
+    const float v = dot3(d, qvec) * inv;
+    if (v < 0.0f || u + v > 1.0f) return -1.0f;
+    return dot3(e2, qvec) * inv;
+}
+
+inline F3 P3(const float* p) { return {p[0], p[1], p[2]}; }
+
+int trace(const void* ap, const float of[3], const float df[3], float eps, int any, float* t_io, uint32_t* prim_out)
+{
+    const Accel& A = *static_cast<const Accel*>(ap);
+    const F3 o = P3(of), d = P3(df);
+    const float tmax = *t_io;
+    const float tmax_eps = tmax + eps;
+    const float win = 4.0f * eps;
+    const bool nan_ray = std::isnan(o.x) || std::isnan(o.y) || std::isnan(o.z) || std::isnan(d.x) ||
+                         std::isnan(d.y) || std::isnan(d.z);
+    struct Cand { uint32_t key; float t; };
+    Cand cand[256];
+    int nc = 0;
+    float tmin = INFINITY;
+    bool occluded = false;
+
+    if (!A.bvh.empty && !nan_ray) {
+        auto inv = [](float v) { return 1.0f / (std::fabs(v) > 1e-20f ? v : std::copysign(1e-20f, v)); };
+        const float ix = inv(d.x), iy = inv(d.y), iz = inv(d.z);
+        const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
+        const float ax = std::fabs(ix), ay = std::fabs(iy), az = std::fabs(iz);
+        // The sphere subtree's boxes are widened by 2e-3 (|o - C| + R): the
+        // reference's sphere test accepts rays up to ~1e-3 |o - c| outside a
+        // sphere from far away (rounding in its discriminant).
+        const F3 oc = sub3(o, P3(A.bvh.sph_c));
+        const float pad = 2e-3f * (std::sqrt(dot3(oc, oc)) + A.bvh.sph_r);
+        const float px = std::fmin(pad * ax, 3e38f), py = std::fmin(pad * ay, 3e38f), pz = std::fmin(pad * az, 3e38f);
+        float cull = any ? tmax_eps : tmax_eps + win;
+        uint32_t stack[128];
+        int sp = 0;
+        uint32_t ref = 0;
+        while (true) {
+            if ((int32_t)ref >= 0) {
+                const vkr::BvhNode& n = A.bvh.nodes[ref];
+                const float cb = cull + std::fabs(cull) * 1e-5f + 1e-5f;
+                float m;
+                std::memcpy(&m, &n.d[2], 4);
+                const float qx = m * px, qy = m * py, qz = m * pz;
+                auto box = [&](float cx, float ex, float cy, float ey, float cz, float ez, float& nr, float& fr) {
+                    const float a0 = std::fma(cx, ix, -ox), a1 = std::fma(ex, ax, qx);
+                    const float a2 = std::fma(cy, iy, -oy), a3 = std::fma(ey, ay, qy);
+                    const float a4 = std::fma(cz, iz, -oz), a5 = std::fma(ez, az, qz);
+                    nr = std::fmax(std::fmax(a0 - a1, a2 - a3), std::fmax(a4 - a5, 0.0f));
+                    fr = std::fmin(std::fmin(a0 + a1, a2 + a3), std::fmin(a4 + a5, cb));
+                };
+                float n0, f0, n1, f1;
+                box(n.a[0], n.a[1], n.a[2], n.a[3], n.b[0], n.b[1], n0, f0);
+                box(n.b[2], n.b[3], n.c[0], n.c[1], n.c[2], n.c[3], n1, f1);
+                const bool h0 = n0 <= f0, h1 = n1 <= f1;
+                if (h0 && h1) {
+                    const bool first0 = n0 <= n1;
+                    stack[sp++] = first0 ? n.d[1] : n.d[0];
+                    ref = first0 ? n.d[0] : n.d[1];
+                } else if (h0 || h1) {
+                    ref = h0 ? n.d[0] : n.d[1];
+                } else {
+                    if (sp == 0) break;
+                    ref = stack[--sp];
+                }
+                continue;
+            }
+            // Leaf: its primitives, from the scan-order arrays.
+            const uint32_t count = ((ref >> vkr::kRefCountShift) & 7u) + 1u;
+            const uint32_t first = ref & vkr::kRefFirstMask;
+            for (uint32_t k = 0; k < count; ++k) {
+                float t;
+                uint32_t key;
+                if (ref & vkr::kRefSphere) {
+                    const uint32_t i = A.bvh.sph_key[first + k];
+                    t = sphere_t(o, d, P3(A.sph[i].P), A.sph[i].r);
+                    key = 0x80000000u | i;
+                } else {
+                    const uint32_t i = A.bvh.tris[first + k].key;
+                    const otri& tr = A.tris[i];
+                    t = tri_t(o, d, P3(tr.v0), P3(tr.v1), P3(tr.v2), eps);
+                    key = i;
+                }
+                if (!(t > eps)) continue;
+                if (any) {
+                    if (t < tmax_eps) occluded = true;
+                } else if (t < std::fmin(tmin, tmax_eps) + win) {
+                    tmin = std::fmin(tmin, t);
+                    int w = 0;   // drop candidates the smaller tmin ruled out
+                    for (int c = 0; c < nc; ++c)
+                        if (cand[c].t < tmin + win) cand[w++] = cand[c];
+                    nc = w;
+                    if (nc < 256) cand[nc++] = Cand{key, t};
+                    cull = std::fmin(tmin, tmax_eps) + win;
+                }
+            }
+            if (occluded) break;
+            if (sp == 0) break;
+            ref = stack[--sp];
+        }
+    }
+    if (any) {
+        if (!occluded && !nan_ray) {
+            for (const opln& p : A.pln) {
+                const float t = plane_t(o, d, P3(p.N), p.len);
+                if ((t > eps) && (t < tmax - eps)) { occluded = true; break; }
+            }
+        }
+        return occluded ? 1 : 0;
+    }
+    // Replay the scan's rule over the candidates in scan order.
+    std::sort(cand, cand + nc, [](const Cand& a, const Cand& b) { return a.key < b.key; });
+    float best = tmax;
+    uint32_t win_key = 0xFFFFFFFFu;
+    for (int c = 0; c < nc; ++c)
+        if (cand[c].t < best + eps) {
+            best = cand[c].t;
+            win_key = cand[c].key;
+        }
+    uint32_t prim = 0xFFFFFFFFu;
+    if (win_key != 0xFFFFFFFFu) prim = (win_key & 0x80000000u) ? ((1u << 30) | (win_key & 0x7FFFFFFFu)) : win_key;
+    if (!nan_ray) {
+        for (uint32_t i = 0; i < A.pln.size(); ++i) {
+            const float t = plane_t(o, d, P3(A.pln[i].N), A.pln[i].len);
+            if ((t > eps) && (t < best - eps)) {
+                best = t;
+                prim = (2u << 30) | i;
+            }
+        }
+    }
+    if (prim == 0xFFFFFFFFu) return 0;
+    *t_io = best;
+    *prim_out = prim;
+    return 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Builds the GPU's BVH over a scene (the arrays are copied) and returns the
+// accelerator to put in oscene.accel, with vkro_accel_fn() in oscene.accel_fn.
+// Same builder settings as vkr_scene_upload with default options (leaf size
+// from the emitter count, depth cap 32, default SAH cost).
+void* vkro_accel_build(const oscene* s)
+{
+    Accel* A = new Accel();
+    A->tris.assign(s->tris, s->tris + s->n_tris);
+    A->sph.assign(s->spheres, s->spheres + s->n_spheres);
+    A->pln.assign(s->planes, s->planes + s->n_planes);
+    uint32_t n_em = 0;
+    for (const osph& q : A->sph)
+        if (!(q.mat.emissive[0] == 0.0f && q.mat.emissive[1] == 0.0f && q.mat.emissive[2] == 0.0f)) ++n_em;
+    vkr::BuildInput in;
+    in.tri_v = A->tris.empty() ? nullptr : reinterpret_cast<const float*>(A->tris.data());
+    in.tri_mat = nullptr;   // materials are resolved by the oracle from the winner's id
+    in.n_tris = s->n_tris;
+    in.sph = A->sph.empty() ? nullptr : reinterpret_cast<const float*>(A->sph.data());
+    in.n_spheres = s->n_spheres;
+    in.max_leaf = vkr::default_leaf(n_em);
+    A->bvh = vkr::build_bvh(in);
+    return A;
+}
+
+void vkro_accel_free(void* a) { delete static_cast<Accel*>(a); }
+
+oaccel_fn vkro_accel_fn(void) { return &trace; }
+
+}  // extern "C"

@@ -273,7 +273,49 @@ static inline const omat* tri_material(const oscene* s, uint32_t i)
     return s->tri_material;
 }
 
+static int trace_ray_scan(const oscene* s, v3 o, v3 d, isect_t* it, float eps);
+
+/* The scan, or the same-BVH baseline's accelerated equivalent when the scene
+ * carries one (it returns the scan's winner; the record is filled here the
+ * way the scan fills it).  `any`: the caller only needs whether anything is
+ * accepted (shadow rays, Tracer.comp:473). */
+static int trace_ray_any(const oscene* s, v3 o, v3 d, isect_t* it, float eps, int any)
+{
+    if (!s->accel_fn) return trace_ray_scan(s, o, d, it, eps);
+    const float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
+    float t = it->t;
+    uint32_t prim = 0xFFFFFFFFu;
+    if (!s->accel_fn(s->accel, of, df, eps, any, &t, &prim)) return 0;
+    if (any) return 1;
+    const uint32_t kind = prim >> 30, i = prim & 0x3FFFFFFFu;
+    it->t = t;
+    it->P = vadd(o, vmuls(d, t));
+    it->prim = prim;
+    if (kind == 0) {
+        const otri* tr = &s->tris[i];
+        const v3 v0 = Vp(tr->v0), v1 = Vp(tr->v1), v2 = Vp(tr->v2);
+        const v3 u = vsub(v1, v0);
+        const v3 v = vsub(v2, v0);
+        it->mat = tri_material(s, i);
+        it->N = V((u.y * v.z) - (u.z * v.y), (u.z * v.x) - (u.x * v.z), (u.x * v.y) - (u.y * v.x));
+    } else if (kind == 1) {
+        const osph* sp = &s->spheres[i];
+        it->mat = &sp->mat;
+        it->N = vdivs(vsub(it->P, Vp(sp->P)), sp->r);
+    } else {
+        const opln* pl = &s->planes[i];
+        it->mat = &pl->mat;
+        it->N = Vp(pl->N);
+    }
+    return 1;
+}
+
 static int trace_ray(const oscene* s, v3 o, v3 d, isect_t* it, float eps)
+{
+    return trace_ray_any(s, o, d, it, eps, 0);
+}
+
+static int trace_ray_scan(const oscene* s, v3 o, v3 d, isect_t* it, float eps)
 {
     int found = 0;
     for (uint32_t i = 0; i < s->n_tris; ++i) {                              /* :378-396 */
@@ -427,7 +469,7 @@ static v3 radiance(path_ctx* pc, rng_t* rng, v3 ro, v3 rd, uint16_t* decision)
                 memset(&sh, 0, sizeof(sh));
                 sh.t = t;
                 pc->rays++;
-                if (!trace_ray(s, it.P, L, &sh, pc->eps))
+                if (!trace_ray_any(s, it.P, L, &sh, pc->eps, 1))
                     e = vadd(e, nee_contribution(it.P, it.N, Vp(pc->f->camera.pos), mat, sp, t, L));
             }
 

@@ -50,6 +50,13 @@ typedef struct oframe {         /* Tracer.comp:151-166, GraphicsDevice.h:20-29 *
     ocam camera;
 } oframe;
 
+/* An accelerated trace_ray for the same-BVH CPU baseline (bvh_accel.cpp):
+ * the winner of the reference scan for ray (o, d) with best = *t_io on entry,
+ * as its primitive id and t, or when `any` is set only whether anything is
+ * accepted.  Returns 1 when something is accepted. */
+typedef int (*oaccel_fn)(const void* accel, const float o[3], const float d[3], float eps, int any,
+                         float* t_io, uint32_t* prim);
+
 typedef struct oscene {
     const otri* tris;    uint32_t n_tris;
     const osph* spheres; uint32_t n_spheres;
@@ -57,6 +64,10 @@ typedef struct oscene {
     const omat* tri_material;            /* material of every triangle (reference: mirror) */
     const omat* materials; uint32_t n_materials;
     uint32_t flags;                      /* bit 0: triangles use materials[mat_index] */
+    /* Optional (NULL: the literal linear scan).  Set only by the CPU
+     * baseline's variant (ii), which traces the same BVH as the GPU. */
+    oaccel_fn accel_fn;
+    const void* accel;
 } oscene;
 
 enum { OR_FLAG_RECORD = 1u, OR_FLAG_REPLAY = 2u };

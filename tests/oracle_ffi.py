@@ -14,6 +14,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+ACCEL_LIB = os.path.join(ORACLE_DIR, "libbvhaccel.so")
 
 KERNEL_PATH, KERNEL_WHITTED = 0, 1
 FLAG_RECORD, FLAG_REPLAY = 1, 2
@@ -22,7 +23,8 @@ FLAG_RECORD, FLAG_REPLAY = 1, 2
 class OScene(C.Structure):
     _fields_ = [("tris", C.c_void_p), ("n_tris", C.c_uint32), ("spheres", C.c_void_p), ("n_spheres", C.c_uint32),
                 ("planes", C.c_void_p), ("n_planes", C.c_uint32), ("tri_material", C.c_void_p),
-                ("materials", C.c_void_p), ("n_materials", C.c_uint32), ("flags", C.c_uint32)]
+                ("materials", C.c_void_p), ("n_materials", C.c_uint32), ("flags", C.c_uint32),
+                ("accel_fn", C.c_void_p), ("accel", C.c_void_p)]
 
 
 class OParams(C.Structure):
@@ -94,6 +96,26 @@ def load():
     return _lib
 
 
+_accel_lib = None
+
+
+def load_accel():
+    """oracle/libbvhaccel.so: the same-BVH traversal used by the CPU baseline."""
+    global _accel_lib
+    if _accel_lib is None:
+        if not os.path.exists(ACCEL_LIB):
+            build()
+        lib = C.CDLL(ACCEL_LIB)
+        lib.vkro_accel_build.restype = C.c_void_p
+        lib.vkro_accel_build.argtypes = [C.POINTER(OScene)]
+        lib.vkro_accel_free.restype = None
+        lib.vkro_accel_free.argtypes = [C.c_void_p]
+        lib.vkro_accel_fn.restype = C.c_void_p
+        lib.vkro_accel_fn.argtypes = []
+        _accel_lib = lib
+    return _accel_lib
+
+
 def f3(v):
     return F3(*[float(x) for x in v])
 
@@ -103,9 +125,12 @@ def _ptr(a):
 
 
 class OracleScene:
-    """Holds references to the numpy arrays so the pointers stay valid."""
+    """Holds references to the numpy arrays so the pointers stay valid.
 
-    def __init__(self, scene):
+    accel=True traces the product's BVH (oracle/bvh_accel.cpp) instead of the
+    reference's linear scan; the results are the same."""
+
+    def __init__(self, scene, accel=False):
         from vkr import MATERIAL, PLANE, SPHERE, TRIANGLE
         self.tris = np.ascontiguousarray(scene.tris, TRIANGLE)
         self.sph = np.ascontiguousarray(scene.spheres, SPHERE)
@@ -114,16 +139,30 @@ class OracleScene:
         self.mats = np.ascontiguousarray(scene.materials, MATERIAL)
         self.c = OScene(_ptr(self.tris), len(self.tris), _ptr(self.sph), len(self.sph), _ptr(self.pln),
                         len(self.pln), self.tm.ctypes.data, _ptr(self.mats), len(self.mats), scene.flags)
+        self._accel = None
+        if accel:
+            lib = load_accel()
+            self._accel_lib = lib
+            self._accel = lib.vkro_accel_build(C.byref(self.c))
+            self.c.accel = self._accel
+            self.c.accel_fn = lib.vkro_accel_fn()
+
+    def __del__(self):
+        if getattr(self, "_accel", None):
+            self._accel_lib.vkro_accel_free(self._accel)
+            self._accel = None
 
 
 def trace(scene, frame, width, height, spp=1, max_depth=1, epsilon=1e-3, kernel=KERNEL_PATH, flags=0,
-          pixels=None, threads=None, decisions=None):
+          pixels=None, threads=None, decisions=None, accel=False, oscene=None):
     """Runs the oracle over `pixels` (flat y*W+x indices; None = full frame).
 
+    accel=True traces the product's BVH instead of the linear scan (same
+    results); oscene reuses an OracleScene built before.
     Returns a dict of numpy outputs indexed like `pixels`."""
     from vkr import FRAME
     lib = load()
-    sc = OracleScene(scene)
+    sc = oscene if oscene is not None else OracleScene(scene, accel=accel)
     fr = np.ascontiguousarray(np.asarray(frame, FRAME).reshape(1))
     n = width * height if pixels is None else len(pixels)
     px = None if pixels is None else np.ascontiguousarray(pixels, np.uint32)

@@ -1,0 +1,109 @@
+"""The same-BVH CPU baseline (oracle/bvh_accel.cpp) against the linear scan.
+
+The accelerated oracle traces the product's BVH (vk-renderer_amd/csrc/bvh.cpp)
+on the CPU and replays the reference's scan-order acceptance rule
+(Assets/Tracer.comp:374-431) over the hits near the closest one; the shadow
+rays use the any-hit form.  It must give exactly the linear scan's results,
+or the CPU baseline it times would not be the same computation.
+"""
+import numpy as np
+import pytest
+
+import oracle_ffi
+import scenes
+import vkr
+
+OUT_KEYS = ("rgba8", "radiance", "prim_id", "hit_t", "ray_count")
+
+
+def _small_soup(seed=0, n_tris=3000, width=96, height=54):
+    return scenes.soup100k(seed=seed, n_tris=n_tris, width=width, height=height, spp=2, max_depth=4,
+                           n_spheres=48)
+
+
+CASES = {
+    "soup3000": lambda: _small_soup(),
+    "many_emitters": lambda: scenes.many_emitters(),
+    "far_sphere": lambda: scenes.far_sphere(res=64),
+    "heightfield": lambda: scenes.heightfield(n=32, extent=64.0, n_spheres=64, width=80, height=60, spp=2, max_depth=4),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_accel_frame_equals_scan(name):
+    w = CASES[name]()
+    kw = dict(spp=w.spp, max_depth=w.max_depth)
+    ref = oracle_ffi.trace(w.scene, w.frame, w.width, w.height, **kw)
+    acc = oracle_ffi.trace(w.scene, w.frame, w.width, w.height, accel=True, **kw)
+    for k in OUT_KEYS:
+        assert np.array_equal(ref[k], acc[k]), f"{name}: {k} differs at {np.argwhere(ref[k] != acc[k])[:5]}"
+
+
+def test_accel_spheres64_pixels_equal_scan():
+    w = scenes.spheres64()
+    px = np.random.default_rng(5).choice(w.width * w.height, 4096, replace=False).astype(np.uint32)
+    ref = oracle_ffi.trace(w.scene, w.frame, w.width, w.height, spp=w.spp, max_depth=w.max_depth, pixels=px)
+    acc = oracle_ffi.trace(w.scene, w.frame, w.width, w.height, spp=w.spp, max_depth=w.max_depth, pixels=px,
+                           accel=True)
+    for k in OUT_KEYS:
+        assert np.array_equal(ref[k], acc[k]), k
+
+
+def test_accel_trace_ray_equals_scan():
+    """Single closest-hit queries: random origins and directions, random
+    t_max (some below the nearest hit), and rays grazing coincident
+    triangles, where the scan's `t < best + EPSILON` rule picks a later
+    primitive than the nearest one."""
+    lib = oracle_ffi.load()
+    rng = np.random.default_rng(11)
+    w = _small_soup(seed=3, n_tris=2000)
+    # Near-coincident copies: the same triangles shifted by less than EPSILON.
+    tris = w.scene.tris
+    dup = tris[:300].copy()
+    for k in ("v0", "v1", "v2"):
+        dup[k] = dup[k] + np.float32(4e-4)
+    scene = vkr.Scene(tris=np.concatenate([tris, dup]), spheres=w.scene.spheres, planes=w.scene.planes)
+    plain = oracle_ffi.OracleScene(scene)
+    fast = oracle_ffi.OracleScene(scene, accel=True)
+    P = oracle_ffi.C.POINTER
+    fp = P(oracle_ffi.C.c_float)
+    n_diff, n_hit = 0, 0
+    for i in range(6000):
+        if i % 3 == 0:   # aim at a duplicated triangle's centroid
+            t = tris[rng.integers(300)]
+            c = (t["v0"] + t["v1"] + t["v2"]) / np.float32(3)
+            o = (rng.uniform(-80, 80, 3)).astype(np.float32)
+            d = (c - o).astype(np.float32)
+        else:
+            o = rng.uniform(-60, 60, 3).astype(np.float32)
+            d = rng.normal(size=3).astype(np.float32)
+        d = (d / np.float32(np.sqrt(np.float32(np.dot(d, d))))).astype(np.float32)
+        tmax = np.float32(rng.choice([1e30, rng.uniform(1.0, 120.0)]))
+        res = []
+        for sc in (plain, fast):
+            prim = oracle_ffi.C.c_uint32(0)
+            tout = oracle_ffi.C.c_float(0)
+            h = lib.vkro_trace_ray(oracle_ffi.C.byref(sc.c), o.ctypes.data_as(fp), d.ctypes.data_as(fp),
+                                   float(tmax), 1e-3, oracle_ffi.C.byref(prim), oracle_ffi.C.byref(tout))
+            res.append((h, prim.value, np.float32(tout.value).view(np.uint32)))
+        n_hit += res[0][0]
+        n_diff += res[0] != res[1]
+    assert n_hit > 1000
+    assert n_diff == 0
+
+
+def test_accel_nan_ray_misses_like_scan():
+    lib = oracle_ffi.load()
+    w = _small_soup(seed=1, n_tris=200)
+    fp = oracle_ffi.C.POINTER(oracle_ffi.C.c_float)
+    o = np.array([np.nan, 0.0, 0.0], np.float32)
+    d = np.array([0.0, 0.0, 1.0], np.float32)
+    out = []
+    for accel in (False, True):
+        sc = oracle_ffi.OracleScene(w.scene, accel=accel)
+        prim = oracle_ffi.C.c_uint32(0)
+        tout = oracle_ffi.C.c_float(0)
+        h = lib.vkro_trace_ray(oracle_ffi.C.byref(sc.c), o.ctypes.data_as(fp), d.ctypes.data_as(fp), 1e30, 1e-3,
+                               oracle_ffi.C.byref(prim), oracle_ffi.C.byref(tout))
+        out.append((h, prim.value))
+    assert out[0] == out[1]
