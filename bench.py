@@ -38,7 +38,7 @@ def parse():
     ap.add_argument("--sched", type=int, default=0, help="0 auto, 1 plain, 2 persistent")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
-    ap.add_argument("--cpu-pixels", type=int, default=131072, help="pixel sample of the same-BVH CPU baseline")
+    ap.add_argument("--cpu-pixels", type=int, default=393216, help="pixel sample of the same-BVH CPU baseline")
     ap.add_argument("--cpu-scan-pixels", type=int, default=64, help="pixel sample of the linear-scan CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
