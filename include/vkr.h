@@ -153,7 +153,15 @@ enum {
     /* Triangles take materials[tri.mat_index] instead of the single
      * tri_material (build extension; the reference shades every triangle
      * with `mirror`, Assets/Tracer.comp:386). */
-    VKR_SCENE_TRI_MATERIAL_INDEX = 1u << 0
+    VKR_SCENE_TRI_MATERIAL_INDEX = 1u << 0,
+    /* Build the triangle subtree on the device (a linear BVH over Morton
+     * codes, one triangle per leaf) instead of the host SAH builder: uploads
+     * of large scenes take milliseconds, traversal is somewhat slower.
+     * max_leaf_size and traversal_cost do not apply to it; when its tree is
+     * deeper than max_depth (0 = 64) the host builder is used instead
+     * (vkr_scene_info.builder says which ran).  Build extension: the
+     * reference has no acceleration structure. */
+    VKR_SCENE_DEVICE_BVH = 1u << 1
 };
 
 typedef struct vkr_scene_desc {
@@ -171,8 +179,9 @@ typedef struct vkr_scene_desc {
 /* Replaces the SceneData SSBO creation and upload
  * (Source/GraphicsDevice.cpp:796-822) and the shader-constant spheres/planes
  * (Assets/Tracer.comp:196-211).  Builds a binned-SAH BVH over triangles and
- * spheres on the host, uploads everything to device-local memory and returns
- * a scene id owned by the context.  Planes stay a linear list. */
+ * spheres on the host (or, with VKR_SCENE_DEVICE_BVH, the triangle subtree on
+ * the device), uploads everything to device-local memory and returns a scene
+ * id owned by the context.  Planes stay a linear list. */
 vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* out_scene);
 
 /* Frees one scene. */
@@ -182,8 +191,12 @@ typedef struct vkr_scene_info {
     uint32_t n_tris, n_spheres, n_planes, n_emitters;
     uint32_t n_nodes, n_leaves, depth, stack_needed;
     uint64_t device_bytes;           /* nodes + primitive records + materials */
-    double   build_ms;               /* host BVH build time */
+    double   build_ms;               /* BVH build time (wall clock, device builds included) */
+    uint32_t builder;                /* VKR_BUILDER_* */
+    uint32_t _pad;
 } vkr_scene_info;
+
+enum { VKR_BUILDER_HOST_SAH = 0u, VKR_BUILDER_DEVICE_LBVH = 1u };
 
 vkr_error vkr_scene_get_info(vkr_ctx* ctx, uint32_t scene, vkr_scene_info* out);
 

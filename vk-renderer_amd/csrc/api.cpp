@@ -24,6 +24,61 @@ using namespace vkr;
 
 namespace vkr {
 thread_local std::string g_err = "no error";
+
+namespace {
+
+// VKR_SCENE_DEVICE_BVH: the triangle subtree built on the device (lbvh.hip)
+// and the sphere subtree on the host, joined under one root node (node 0)
+// whose sphere child is always entered, the layout of the host builder:
+//   [root] [n - 1 triangle nodes] [sphere nodes]   (root only with spheres).
+// On success sc->nodes and sc->tris are filled and `sph` holds the sphere
+// records.  Returns false with *err == hipSuccess when the tree is deeper
+// than depth_cap (nothing is kept; the caller builds on the host).
+bool device_build(Scene* sc, uint32_t n, BuildInput bi, uint32_t depth_cap, BuildOutput& sph, uint32_t& n_nodes,
+                  uint32_t& n_leaves, uint32_t& depth, hipError_t* err)
+{
+    *err = hipSuccess;
+    bi.tri_v = nullptr;
+    bi.tri_mat = nullptr;
+    bi.n_tris = 0;
+    BuildOutput sb = build_bvh(bi);   // spheres only (empty when no sphere is finite)
+    const bool has_sph = !sb.empty;
+    const uint32_t node_off = has_sph ? 1u : 0u;
+    const uint32_t sph_off = node_off + n - 1u;
+    const size_t total = (size_t)sph_off + (has_sph ? sb.nodes.size() : 0u);
+    if (total >= (1u << 26)) {
+        *err = hipErrorInvalidValue;
+        return false;
+    }
+    if ((*err = sc->nodes.alloc(total * sizeof(BvhNode))) != hipSuccess) return false;
+    if ((*err = sc->tris.alloc((size_t)n * sizeof(TriRecord))) != hipSuccess) return false;
+    LbvhRoot root{};
+    *err = lbvh_build(sc->tris_orig.as<vkr_triangle>(), n, node_off, sc->nodes.as<BvhNode>(),
+                      sc->tris.as<TriRecord>(), &root, nullptr);
+    if (*err != hipSuccess) return false;
+    const uint32_t dep = has_sph ? 1u + std::max(root.height, sb.depth) : root.height;
+    if (dep > depth_cap) {
+        sc->nodes.reset();
+        sc->tris.reset();
+        return false;
+    }
+    if (has_sph) {
+        offset_inner_refs(sb.nodes, sph_off);
+        const BvhNode top = join_root(root.lo, root.hi, node_off, sph_off);
+        BvhNode* dn = sc->nodes.as<BvhNode>();
+        if ((*err = hipMemcpy(dn, &top, sizeof top, hipMemcpyHostToDevice)) != hipSuccess) return false;
+        *err = hipMemcpy(dn + sph_off, sb.nodes.data(), sb.nodes.size() * sizeof(BvhNode), hipMemcpyHostToDevice);
+        if (*err != hipSuccess) return false;
+    }
+    n_nodes = (uint32_t)total;
+    n_leaves = n + sb.leaves;
+    depth = dep;
+    sb.nodes.clear();
+    sph = std::move(sb);
+    return true;
+}
+
+}  // namespace
 }  // namespace vkr
 
 extern "C" {
@@ -102,7 +157,9 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
     auto sc = std::make_unique<Scene>();
     sc->host_spheres.assign(desc->spheres, desc->spheres + desc->n_spheres);
     const std::vector<uint32_t> em = emitter_list(sc->host_spheres);
+    VKR_HIP(sc->tris_orig.upload(desc->tris, (size_t)desc->n_tris * sizeof(vkr_triangle)));
     const auto t0 = std::chrono::steady_clock::now();
+    auto t1 = t0;
     BuildInput bi;
     bi.tri_v = reinterpret_cast<const float*>(desc->tris);
     bi.tri_mat = desc->tris ? reinterpret_cast<const uint32_t*>(desc->tris) + 3 : nullptr;
@@ -112,13 +169,37 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
     bi.max_leaf = desc->max_leaf_size ? desc->max_leaf_size : default_leaf((uint32_t)em.size());
     bi.max_depth = desc->max_depth ? desc->max_depth : 32u;
     bi.traversal_cost = desc->traversal_cost;
-    BuildOutput bo = build_bvh(bi);
-    const auto t1 = std::chrono::steady_clock::now();
+    // Host build: bo is the whole tree.  Device build: bo holds the sphere
+    // records only; nodes and triangle records are already on the device.
+    BuildOutput bo;
+    uint32_t builder = VKR_BUILDER_HOST_SAH, n_nodes = 0, n_leaves = 0, depth = 0;
+    bool has_bvh = false;
+    if ((desc->flags & VKR_SCENE_DEVICE_BVH) && desc->n_tris >= 2u) {
+        hipError_t e = hipSuccess;
+        const uint32_t cap = desc->max_depth ? desc->max_depth : 64u;
+        if (device_build(sc.get(), desc->n_tris, bi, cap, bo, n_nodes, n_leaves, depth, &e)) {
+            builder = VKR_BUILDER_DEVICE_LBVH;
+            has_bvh = true;
+            t1 = std::chrono::steady_clock::now();
+        } else if (e != hipSuccess) {
+            return fail_hip("device BVH build", e);
+        }
+    }
+    if (builder == VKR_BUILDER_HOST_SAH) {
+        bo = build_bvh(bi);
+        t1 = std::chrono::steady_clock::now();
+        if (bo.nodes.size() >= (1u << 26))
+            return fail(VKR_UNSUPPORTED, "BVH has 2^26 nodes or more (nodes are addressed with 32-bit offsets)");
+        n_nodes = (uint32_t)bo.nodes.size();
+        n_leaves = bo.leaves;
+        depth = bo.depth;
+        has_bvh = !bo.empty;
+        VKR_HIP(sc->nodes.upload(bo.nodes.data(), bo.nodes.size() * sizeof(BvhNode)));
+        VKR_HIP(sc->tris.upload(bo.tris.data(), bo.tris.size() * sizeof(TriRecord)));
+    }
 
-    const uint32_t stack = bo.empty ? 32u : stack_variant(bo.depth);
+    const uint32_t stack = has_bvh ? stack_variant(depth) : 32u;
     if (stack == 0) return fail(VKR_UNSUPPORTED, "BVH deeper than 64 levels");
-    if (bo.nodes.size() >= (1u << 26))
-        return fail(VKR_UNSUPPORTED, "BVH has 2^26 nodes or more (nodes are addressed with 32-bit offsets)");
 
     // Parameters as uploaded: light scale 1, emission = the sphere's emissive.
     sc->host_theta.assign(7u * desc->n_spheres + 1u, 0.0f);
@@ -133,11 +214,8 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
     sc->host_theta[7u * desc->n_spheres] = 1.0f;
     VKR_HIP(sc->theta.upload(sc->host_theta.data(), sc->host_theta.size() * sizeof(float)));
 
-    VKR_HIP(sc->nodes.upload(bo.nodes.data(), bo.nodes.size() * sizeof(BvhNode)));
-    VKR_HIP(sc->tris.upload(bo.tris.data(), bo.tris.size() * sizeof(TriRecord)));
     VKR_HIP(sc->sph.upload(bo.sph.data(), bo.sph.size() * sizeof(float)));
     VKR_HIP(sc->sph_key.upload(bo.sph_key.data(), bo.sph_key.size() * sizeof(uint32_t)));
-    VKR_HIP(sc->tris_orig.upload(desc->tris, (size_t)desc->n_tris * sizeof(vkr_triangle)));
     VKR_HIP(sc->spheres.upload(desc->spheres, (size_t)desc->n_spheres * sizeof(vkr_sphere)));
     VKR_HIP(sc->planes.upload(desc->planes, (size_t)desc->n_planes * sizeof(vkr_plane)));
     VKR_HIP(sc->tri_mat.upload(desc->tri_material ? desc->tri_material : &kMirror, sizeof(vkr_material)));
@@ -163,7 +241,7 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
     d.emitters_bounded = emitters_bounded(sc->host_spheres, em);
     d.n_materials = nm;
     d.flags = desc->flags;
-    d.has_bvh = bo.empty ? 0u : 1u;
+    d.has_bvh = has_bvh ? 1u : 0u;
     d.stack_needed = stack;
     for (int k = 0; k < 3; ++k) d.sph_c[k] = bo.sph_c[k];
     d.sph_r = bo.sph_r;
@@ -173,14 +251,15 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
     in.n_spheres = desc->n_spheres;
     in.n_planes = desc->n_planes;
     in.n_emitters = (uint32_t)em.size();
-    in.n_nodes = (uint32_t)bo.nodes.size();
-    in.n_leaves = bo.leaves;
-    in.depth = bo.depth;
+    in.n_nodes = n_nodes;
+    in.n_leaves = n_leaves;
+    in.depth = depth;
     in.stack_needed = stack;
     in.device_bytes = sc->nodes.bytes + sc->tris.bytes + sc->sph.bytes + sc->sph_key.bytes + sc->tris_orig.bytes +
                       sc->spheres.bytes + sc->planes.bytes + sc->tri_mat.bytes + sc->materials.bytes +
                       sc->emitters.bytes;
     in.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    in.builder = builder;
 
     // Reuse a freed slot if any.
     for (uint32_t i = 0; i < ctx->scenes.size(); ++i) {

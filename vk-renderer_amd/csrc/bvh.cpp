@@ -301,6 +301,27 @@ inline void pad(float* lo, float* hi)
 
 }  // namespace
 
+BvhNode join_root(const float lo[3], const float hi[3], uint32_t tri_ref, uint32_t sph_ref)
+{
+    BvhNode n{};
+    if (lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2]) {
+        Box b;
+        b.grow(lo, hi);
+        set_child(n, 0, b, tri_ref);
+    } else {
+        set_child_never(n, 0, tri_ref);
+    }
+    set_child_always(n, 1, sph_ref);
+    return n;
+}
+
+void offset_inner_refs(std::vector<BvhNode>& nodes, uint32_t off)
+{
+    for (BvhNode& n : nodes)
+        for (int s = 0; s < 2; ++s)
+            if (!(n.d[s] & kRefLeaf)) n.d[s] += off;
+}
+
 BuildOutput build_bvh(const BuildInput& in)
 {
     BuildOutput out;

@@ -87,6 +87,14 @@ struct BuildOutput {
     bool empty = true;
 };
 
+// The node above a device-built triangle subtree and a sphere subtree:
+// child 0 the triangle box [lo, hi] (an empty box: never entered) with ref
+// tri_ref, child 1 always entered with ref sph_ref.
+BvhNode join_root(const float lo[3], const float hi[3], uint32_t tri_ref, uint32_t sph_ref);
+
+// Adds off to every inner-node ref of nodes (leaf refs are unchanged).
+void offset_inner_refs(std::vector<BvhNode>& nodes, uint32_t off);
+
 // Builds a binned-SAH BVH.  Leaves hold one primitive kind only and at most
 // max_leaf primitives; no leaf is deeper than max_depth (splits fall back to
 // balanced median splits near the cap), so a traversal stack of max_depth

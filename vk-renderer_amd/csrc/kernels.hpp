@@ -135,4 +135,16 @@ uint32_t persistent_grid(const PathVariant& v, int device);
 // Largest stack depth with a compiled kernel variant (0 when none fits).
 uint32_t stack_variant(uint32_t needed);
 
+// Device build of a triangle subtree (lbvh.hip): internal nodes go to
+// d_nodes[node_off .. node_off + n - 2] (its root at node_off), one record
+// per triangle to d_recs (Morton order, leaf j = record j).  Returns, after a
+// stream synchronisation, the exact bounds of the padded triangle boxes and
+// the subtree's height (its deepest leaf, the root's children at 1).
+struct LbvhRoot {
+    float lo[3], hi[3];
+    uint32_t height;
+};
+hipError_t lbvh_build(const vkr_triangle* d_tris, uint32_t n, uint32_t node_off, BvhNode* d_nodes,
+                      TriRecord* d_recs, LbvhRoot* root, hipStream_t s);
+
 }  // namespace vkr

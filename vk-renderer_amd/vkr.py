@@ -41,6 +41,8 @@ SCHED_AUTO, SCHED_PLAIN, SCHED_PERSISTENT = 0, 1, 2
 TRACE_RECORD_DECISIONS, TRACE_REPLAY_DECISIONS = 1, 2
 TRACE_RAY_POOL, TRACE_NO_RAY_POOL = 4, 8
 SCENE_TRI_MATERIAL_INDEX = 1
+SCENE_DEVICE_BVH = 2
+BUILDER_HOST_SAH, BUILDER_DEVICE_LBVH = 0, 1
 CTX_COUNT_WORK = 1
 DEBUG_OPS = {"rand": 0, "tri": 1, "sphere": 2, "plane": 3, "brdf": 4, "jitter": 5, "cos": 6}
 ERRORS = {0: "SUCCESS", 1: "NO_SUITABLE_GPU", 2: "NO_SUITABLE_SURFACE", 3: "UNKNOWN",
@@ -83,7 +85,7 @@ class SceneInfo(C.Structure):
     _fields_ = [("n_tris", C.c_uint32), ("n_spheres", C.c_uint32), ("n_planes", C.c_uint32),
                 ("n_emitters", C.c_uint32), ("n_nodes", C.c_uint32), ("n_leaves", C.c_uint32),
                 ("depth", C.c_uint32), ("stack_needed", C.c_uint32), ("device_bytes", C.c_uint64),
-                ("build_ms", C.c_double)]
+                ("build_ms", C.c_double), ("builder", C.c_uint32), ("_pad", C.c_uint32)]
 
 
 class TraceParams(C.Structure):
@@ -284,14 +286,18 @@ class Context:
     def set_flags(self, flags: int):
         check(self.lib.vkr_set_ctx_flags(self.h, flags), "vkr_set_ctx_flags")
 
-    def upload(self, scene: Scene, max_leaf: int = 0, max_depth: int = 0, traversal_cost: float = 0.0) -> int:
+    def upload(self, scene: Scene, max_leaf: int = 0, max_depth: int = 0, traversal_cost: float = 0.0,
+               device_bvh: bool = False) -> int:
+        """Uploads a scene; device_bvh builds its triangle subtree on the GPU
+        (VKR_SCENE_DEVICE_BVH)."""
         tris = np.ascontiguousarray(scene.tris, TRIANGLE)
         sph = np.ascontiguousarray(scene.spheres, SPHERE)
         pln = np.ascontiguousarray(scene.planes, PLANE)
         tm = np.ascontiguousarray(np.asarray(scene.tri_material, MATERIAL).reshape(1))
         mats = np.ascontiguousarray(scene.materials, MATERIAL)
         d = SceneDesc(_ptr(tris), len(tris), _ptr(sph), len(sph), _ptr(pln), len(pln), _ptr(tm),
-                      _ptr(mats), len(mats), scene.flags, max_leaf, max_depth, traversal_cost)
+                      _ptr(mats), len(mats), scene.flags | (SCENE_DEVICE_BVH if device_bvh else 0), max_leaf,
+                      max_depth, traversal_cost)
         sid = C.c_uint32()
         check(self.lib.vkr_scene_upload(self.h, C.byref(d), C.byref(sid)), "vkr_scene_upload")
         return sid.value
