@@ -53,7 +53,8 @@ bool device_build(Scene* sc, uint32_t n, BuildInput bi, uint32_t depth_cap, Buil
     if ((*err = sc->nodes.alloc(total * sizeof(BvhNode))) != hipSuccess) return false;
     if ((*err = sc->tris.alloc((size_t)n * sizeof(TriRecord))) != hipSuccess) return false;
     LbvhRoot root{};
-    *err = lbvh_build(sc->tris_orig.as<vkr_triangle>(), n, node_off, sc->nodes.as<BvhNode>(),
+    const uint32_t tri_cap = has_sph ? depth_cap - 1u : depth_cap;
+    *err = lbvh_build(sc->tris_orig.as<vkr_triangle>(), n, node_off, tri_cap, sc->nodes.as<BvhNode>(),
                       sc->tris.as<TriRecord>(), &root, nullptr);
     if (*err != hipSuccess) return false;
     const uint32_t dep = has_sph ? 1u + std::max(root.height, sb.depth) : root.height;
