@@ -54,8 +54,8 @@ bool device_build(Scene* sc, uint32_t n, BuildInput bi, uint32_t depth_cap, Buil
     if ((*err = sc->tris.alloc((size_t)n * sizeof(TriRecord))) != hipSuccess) return false;
     LbvhRoot root{};
     const uint32_t tri_cap = has_sph ? depth_cap - 1u : depth_cap;
-    *err = lbvh_build(sc->tris_orig.as<vkr_triangle>(), n, node_off, tri_cap, sc->nodes.as<BvhNode>(),
-                      sc->tris.as<TriRecord>(), &root, nullptr);
+    *err = lbvh_build(sc->tris_orig.as<vkr_triangle>(), n, node_off, bi.max_leaf, tri_cap,
+                      sc->nodes.as<BvhNode>(), sc->tris.as<TriRecord>(), &root, nullptr);
     if (*err != hipSuccess) return false;
     const uint32_t dep = has_sph ? 1u + std::max(root.height, sb.depth) : root.height;
     if (dep > depth_cap) {
@@ -71,8 +71,8 @@ bool device_build(Scene* sc, uint32_t n, BuildInput bi, uint32_t depth_cap, Buil
         *err = hipMemcpy(dn + sph_off, sb.nodes.data(), sb.nodes.size() * sizeof(BvhNode), hipMemcpyHostToDevice);
         if (*err != hipSuccess) return false;
     }
-    n_nodes = (uint32_t)total;
-    n_leaves = n + sb.leaves;
+    n_nodes = root.nodes + (has_sph ? 1u + (uint32_t)sb.nodes.size() : 0u);   // live nodes
+    n_leaves = root.leaves + sb.leaves;
     depth = dep;
     sb.nodes.clear();
     sph = std::move(sb);

@@ -137,16 +137,17 @@ uint32_t stack_variant(uint32_t needed);
 
 // Device build of a triangle subtree (lbvh.hip): internal nodes go to
 // d_nodes[node_off .. node_off + n - 2] (its root at node_off), one record
-// per triangle to d_recs (Morton order, leaf j = record j).  Returns, after a
-// stream synchronisation, the subtree's height (its deepest leaf, the root's
-// children at 1) and the exact bounds of the padded triangle boxes.  A tree
-// higher than height_cap is rejected after its topology: only the height is
-// returned and the nodes' boxes and the records are not written.
+// per triangle to d_recs (Morton order; a leaf is a run of at most max_leaf
+// records).  Returns, after a stream synchronisation, the subtree's height
+// (its deepest leaf, the root's children at 1), its live node and leaf
+// counts, and the exact bounds of the padded triangle boxes.  A tree higher
+// than height_cap is rejected after its topology: only the height and counts
+// are returned and the nodes' boxes and the records are not written.
 struct LbvhRoot {
     float lo[3], hi[3];
-    uint32_t height;
+    uint32_t height, nodes, leaves;
 };
-hipError_t lbvh_build(const vkr_triangle* d_tris, uint32_t n, uint32_t node_off, uint32_t height_cap,
-                      BvhNode* d_nodes, TriRecord* d_recs, LbvhRoot* root, hipStream_t s);
+hipError_t lbvh_build(const vkr_triangle* d_tris, uint32_t n, uint32_t node_off, uint32_t max_leaf,
+                      uint32_t height_cap, BvhNode* d_nodes, TriRecord* d_recs, LbvhRoot* root, hipStream_t s);
 
 }  // namespace vkr

@@ -14,7 +14,9 @@
 // written by an earlier launch.  (The usual bottom-up climb with per-node
 // atomics needs device-scope fences at every step, which on gfx950 write
 // back and invalidate the L2 of each XCD: 17 ms instead of 0.3 ms on 2M
-// triangles.)  Leaves hold one triangle.
+// triangles.)  A subtree of at most max_leaf triangles (a contiguous range of
+// records) becomes one leaf: its parent refers to the records directly and
+// its own nodes stay unused ("dead") in the array.
 //
 // Output, in the layout of bvh.hpp: internal node i of the Karras tree is node
 // node_off + i (node 0 of the tree is its root); leaf j is triangle record j,
@@ -170,8 +172,10 @@ __global__ __launch_bounds__(kBuildBlock) void k_reduce_bounds(const uint32_t* _
         for (int k = 0; k < 6; ++k) cb[k] = v[k];
 }
 
-// Morton code of each box centre over the centre bounds; non-finite
-// triangles get the largest key and sort last.
+// Morton code of each box centre over the centre bounds, one scale for all
+// three axes (the largest extent), so a flat scene is not split along its
+// thin axis as often as along the others; non-finite triangles get the
+// largest key and sort last.
 __global__ __launch_bounds__(kBuildBlock) void k_morton(uint32_t n, const float4* __restrict__ blo,
                                                           const float4* __restrict__ bhi, const uint32_t* cb,
                                                           uint64_t* __restrict__ keys, uint32_t* __restrict__ vals)
@@ -185,11 +189,12 @@ __global__ __launch_bounds__(kBuildBlock) void k_morton(uint32_t n, const float4
         return;
     }
     const float c[3] = {0.5f * (l.x + h.x), 0.5f * (l.y + h.y), 0.5f * (l.z + h.z)};
+    const float lo[3] = {unord(cb[0]), unord(cb[1]), unord(cb[2])};
+    const float ext = fmaxf(fmaxf(unord(cb[3]) - lo[0], unord(cb[4]) - lo[1]), unord(cb[5]) - lo[2]);
+    const float s = ext > 0.0f ? 2097151.0f / ext : 0.0f;
     uint64_t key = 0;
     for (int k = 0; k < 3; ++k) {
-        const float lo = unord(cb[k]), ext = unord(cb[3 + k]) - lo;
-        const float s = ext > 0.0f ? 2097151.0f / ext : 0.0f;
-        const float q = fminf(fmaxf((c[k] - lo) * s, 0.0f), 2097151.0f);
+        const float q = fminf(fmaxf((c[k] - lo[k]) * s, 0.0f), 2097151.0f);
         key |= spread3((uint64_t)q) << (2 - k);
     }
     keys[i] = key;
@@ -206,9 +211,12 @@ __device__ __forceinline__ int delta(const uint64_t* __restrict__ k, int n, int 
 }
 
 // Internal node i: its key range, split and children (written to the node's
-// refs); par_in[c] = parent of internal node c.
+// refs; a child range of at most max_leaf records becomes a leaf ref);
+// par_in[c] = parent of internal node c; alive[i] = 0 when node i lies in a
+// collapsed range (then nothing refers to it).
 __global__ __launch_bounds__(kBuildBlock) void k_topology(int n, const uint64_t* __restrict__ k, uint32_t node_off,
-                                                            BvhNode* __restrict__ nodes, uint32_t* __restrict__ par_in)
+                                                            uint32_t max_leaf, BvhNode* __restrict__ nodes,
+                                                            uint32_t* __restrict__ par_in, uint32_t* __restrict__ alive)
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n - 1) return;
@@ -228,19 +236,16 @@ __global__ __launch_bounds__(kBuildBlock) void k_topology(int n, const uint64_t*
     } while (t > 1);
     const int g = i + s * d + (d < 0 ? -1 : 0);
     const int lo = i < j ? i : j, hi = i < j ? j : i;
-    uint32_t r0, r1;
-    if (lo == g) {
-        r0 = kRefLeaf | (uint32_t)g;
-    } else {
-        r0 = node_off + (uint32_t)g;
-        par_in[g] = (uint32_t)i;
-    }
-    if (hi == g + 1) {
-        r1 = kRefLeaf | (uint32_t)(g + 1);
-    } else {
-        r1 = node_off + (uint32_t)(g + 1);
-        par_in[g + 1] = (uint32_t)i;
-    }
+    // Children: [lo, g] and [g + 1, hi]; a range of one record is always a
+    // leaf (and not a Karras internal node).
+    const uint32_t nl = (uint32_t)(g - lo + 1), nr = (uint32_t)(hi - g);
+    const uint32_t r0 = nl <= max_leaf ? (kRefLeaf | ((nl - 1u) << kRefCountShift) | (uint32_t)lo)
+                                       : node_off + (uint32_t)g;
+    const uint32_t r1 = nr <= max_leaf ? (kRefLeaf | ((nr - 1u) << kRefCountShift) | (uint32_t)(g + 1))
+                                       : node_off + (uint32_t)(g + 1);
+    if (nl > 1u) par_in[g] = (uint32_t)i;
+    if (nr > 1u) par_in[g + 1] = (uint32_t)i;
+    alive[i] = (i == 0 || (uint32_t)(hi - lo + 1) > max_leaf) ? 1u : 0u;
     BvhNode* nd = nodes + node_off + i;
     nd->d[0] = r0;
     nd->d[1] = r1;
@@ -249,45 +254,80 @@ __global__ __launch_bounds__(kBuildBlock) void k_topology(int n, const uint64_t*
 }
 
 constexpr uint32_t kDepthLimit = 255;   // deeper trees are reported as 255 (and rejected)
+constexpr uint32_t kDead = 0xFFFFFFFFu;  // depth of a node in a collapsed range (no level)
 
-// Depth of internal node i below the root (the root, node 0, is at 0), by
-// following parents; per block the maximum goes to part[b].
-__global__ __launch_bounds__(kBuildBlock) void k_depth(uint32_t ni, const uint32_t* __restrict__ par_in,
+// Depth of live internal node i below the root (the root, node 0, is at 0),
+// by following parents (a live node's parent is live).  Per block, to
+// part[3 b ..]: the deepest live node, the live nodes and their leaf refs.
+__global__ __launch_bounds__(kBuildBlock) void k_depth(uint32_t ni, uint32_t node_off,
+                                                         const BvhNode* __restrict__ nodes,
+                                                         const uint32_t* __restrict__ par_in,
+                                                         const uint32_t* __restrict__ alive,
                                                          uint32_t* __restrict__ depth, uint32_t* __restrict__ part)
 {
-    __shared__ uint32_t red[kBuildBlock / 64];
+    __shared__ uint32_t red[kBuildBlock / 64][3];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t d = 0;
+    uint32_t v[3] = {0u, 0u, 0u};
     if (i < ni) {
-        uint32_t p = i;
-        while (p != 0u && d < kDepthLimit) {
-            p = par_in[p];
-            ++d;
+        if (alive[i]) {
+            uint32_t p = i, d = 0;
+            while (p != 0u && d < kDepthLimit) {
+                p = par_in[p];
+                ++d;
+            }
+            depth[i] = d;
+            const BvhNode& nd = nodes[node_off + i];
+            v[0] = d;
+            v[1] = 1u;
+            v[2] = ((nd.d[0] & kRefLeaf) ? 1u : 0u) + ((nd.d[1] & kRefLeaf) ? 1u : 0u);
+        } else {
+            depth[i] = kDead;
         }
-        depth[i] = d;
     }
-    for (int off = 32; off > 0; off >>= 1) d = max(d, (uint32_t)__shfl_xor((int)d, off));
-    if ((threadIdx.x & 63u) == 0u) red[threadIdx.x >> 6] = d;
+    for (int off = 32; off > 0; off >>= 1) {
+        v[0] = max(v[0], (uint32_t)__shfl_xor((int)v[0], off));
+        v[1] += (uint32_t)__shfl_xor((int)v[1], off);
+        v[2] += (uint32_t)__shfl_xor((int)v[2], off);
+    }
+    if ((threadIdx.x & 63u) == 0u)
+        for (int k = 0; k < 3; ++k) red[threadIdx.x >> 6][k] = v[k];
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (uint32_t j = 1; j < blockDim.x / 64u; ++j) d = max(d, red[j]);
-        part[blockIdx.x] = d;
+        for (uint32_t j = 1; j < blockDim.x / 64u; ++j) {
+            v[0] = max(v[0], red[j][0]);
+            v[1] += red[j][1];
+            v[2] += red[j][2];
+        }
+        for (int k = 0; k < 3; ++k) part[blockIdx.x * 3u + k] = v[k];
     }
 }
 
-// One block: the maximum of nb partials to out[0].
-__global__ __launch_bounds__(kBuildBlock) void k_reduce_max(const uint32_t* __restrict__ part, uint32_t nb,
-                                                              uint32_t* __restrict__ out)
+// One block: reduces nb (max, sum, sum) partials into out[0..2].
+__global__ __launch_bounds__(kBuildBlock) void k_reduce3(const uint32_t* __restrict__ part, uint32_t nb,
+                                                           uint32_t* __restrict__ out)
 {
-    __shared__ uint32_t red[kBuildBlock / 64];
-    uint32_t d = 0;
-    for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x) d = max(d, part[j]);
-    for (int off = 32; off > 0; off >>= 1) d = max(d, (uint32_t)__shfl_xor((int)d, off));
-    if ((threadIdx.x & 63u) == 0u) red[threadIdx.x >> 6] = d;
+    __shared__ uint32_t red[kBuildBlock / 64][3];
+    uint32_t v[3] = {0u, 0u, 0u};
+    for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x) {
+        v[0] = max(v[0], part[j * 3u]);
+        v[1] += part[j * 3u + 1u];
+        v[2] += part[j * 3u + 2u];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        v[0] = max(v[0], (uint32_t)__shfl_xor((int)v[0], off));
+        v[1] += (uint32_t)__shfl_xor((int)v[1], off);
+        v[2] += (uint32_t)__shfl_xor((int)v[2], off);
+    }
+    if ((threadIdx.x & 63u) == 0u)
+        for (int k = 0; k < 3; ++k) red[threadIdx.x >> 6][k] = v[k];
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (uint32_t j = 1; j < blockDim.x / 64u; ++j) d = max(d, red[j]);
-        out[0] = d;
+        for (uint32_t j = 1; j < blockDim.x / 64u; ++j) {
+            v[0] = max(v[0], red[j][0]);
+            v[1] += red[j][1];
+            v[2] += red[j][2];
+        }
+        for (int k = 0; k < 3; ++k) out[k] = v[k];
     }
 }
 
@@ -309,9 +349,14 @@ __global__ __launch_bounds__(kBuildBlock) void k_level(uint32_t ni, uint32_t lev
         const uint32_t ref = nd->d[slot];
         float4 l, h;
         if (ref & kRefLeaf) {
-            const uint32_t t = order[ref & kRefFirstMask];
-            l = blo[t];
-            h = bhi[t];
+            const uint32_t first = ref & kRefFirstMask, cnt = ((ref >> kRefCountShift) & 7u) + 1u;
+            l = blo[order[first]];
+            h = bhi[order[first]];
+            for (uint32_t c = 1; c < cnt; ++c) {
+                const float4 l2 = blo[order[first + c]], h2 = bhi[order[first + c]];
+                l = make_float4(fminf(l.x, l2.x), fminf(l.y, l2.y), fminf(l.z, l2.z), 0.0f);
+                h = make_float4(fmaxf(h.x, h2.x), fmaxf(h.y, h2.y), fmaxf(h.z, h2.z), 0.0f);
+            }
         } else {
             l = nlo[ref - node_off];
             h = nhi[ref - node_off];
@@ -361,10 +406,10 @@ inline size_t up256(size_t b) { return (b + 255u) & ~(size_t)255u; }
 
 }  // namespace
 
-hipError_t lbvh_build(const vkr_triangle* d_tris, uint32_t n, uint32_t node_off, uint32_t height_cap,
-                      BvhNode* d_nodes, TriRecord* d_recs, LbvhRoot* root, hipStream_t s)
+hipError_t lbvh_build(const vkr_triangle* d_tris, uint32_t n, uint32_t node_off, uint32_t max_leaf,
+                      uint32_t height_cap, BvhNode* d_nodes, TriRecord* d_recs, LbvhRoot* root, hipStream_t s)
 {
-    if (n < 2u) return hipErrorInvalidValue;
+    if (n < 2u || max_leaf < 1u || max_leaf > kMaxLeaf) return hipErrorInvalidValue;
     const uint32_t ni = n - 1u;
     const uint32_t gn = (n + kBuildBlock - 1u) / kBuildBlock;
     const uint32_t gi = (ni + kBuildBlock - 1u) / kBuildBlock;
@@ -380,8 +425,8 @@ hipError_t lbvh_build(const vkr_triangle* d_tris, uint32_t n, uint32_t node_off,
     const size_t b_u32n = up256((size_t)n * sizeof(uint32_t));
     const size_t b_u32i = up256((size_t)ni * sizeof(uint32_t));
     const size_t b_part = up256((size_t)gn * 6u * sizeof(uint32_t));
-    const size_t b_part2 = up256((size_t)gi * sizeof(uint32_t));
-    const size_t total = 2 * b_f4n + 2 * b_f4i + 2 * b_key + 2 * b_u32n + 2 * b_u32i + b_part + b_part2 + 256 +
+    const size_t b_part2 = up256((size_t)gi * 3u * sizeof(uint32_t));
+    const size_t total = 2 * b_f4n + 2 * b_f4i + 2 * b_key + 2 * b_u32n + 3 * b_u32i + b_part + b_part2 + 256 +
                          up256(sort_bytes);
     Scratch sc;
     if ((e = hipMalloc(&sc.p, total)) != hipSuccess) return e;
@@ -401,9 +446,10 @@ hipError_t lbvh_build(const vkr_triangle* d_tris, uint32_t n, uint32_t node_off,
     uint32_t* order = (uint32_t*)take(b_u32n);
     uint32_t* par_in = (uint32_t*)take(b_u32i);
     uint32_t* depth = (uint32_t*)take(b_u32i);
+    uint32_t* alive = (uint32_t*)take(b_u32i);
     uint32_t* part = (uint32_t*)take(b_part);
     uint32_t* part2 = (uint32_t*)take(b_part2);
-    uint32_t* small = (uint32_t*)take(256);   // [0..5] centre bounds, [6] deepest internal node
+    uint32_t* small = (uint32_t*)take(256);   // [0..5] centre bounds, [6..8] deepest, live nodes, leaves
     void* sort_tmp = take(up256(sort_bytes));
 
     const dim3 blk(kBuildBlock);
@@ -413,14 +459,17 @@ hipError_t lbvh_build(const vkr_triangle* d_tris, uint32_t n, uint32_t node_off,
     if ((e = hipGetLastError()) != hipSuccess) return e;
     e = hipcub::DeviceRadixSort::SortPairs(sort_tmp, sort_bytes, keys, keys_s, vals, order, (int)n, 0, 64, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_topology, dim3(gi), blk, 0, s, (int)n, keys_s, node_off, d_nodes, par_in);
-    hipLaunchKernelGGL(k_depth, dim3(gi), blk, 0, s, ni, par_in, depth, part2);
-    hipLaunchKernelGGL(k_reduce_max, dim3(1), blk, 0, s, part2, gi, small + 6);
+    hipLaunchKernelGGL(k_topology, dim3(gi), blk, 0, s, (int)n, keys_s, node_off, max_leaf, d_nodes, par_in, alive);
+    hipLaunchKernelGGL(k_depth, dim3(gi), blk, 0, s, ni, node_off, d_nodes, par_in, alive, depth, part2);
+    hipLaunchKernelGGL(k_reduce3, dim3(1), blk, 0, s, part2, gi, small + 6);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    uint32_t maxd = 0;
-    if ((e = hipMemcpyAsync(&maxd, small + 6, sizeof maxd, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    uint32_t st[3] = {0u, 0u, 0u};
+    if ((e = hipMemcpyAsync(st, small + 6, sizeof st, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-    root->height = maxd + 1u;   // the deepest internal node's children are leaves
+    const uint32_t maxd = st[0];
+    root->height = maxd + 1u;   // the deepest live node's children are leaves
+    root->nodes = st[1];
+    root->leaves = st[2];
     if (root->height > height_cap) return hipSuccess;   // rejected: no boxes, no records
     for (uint32_t level = maxd + 1u; level-- > 0u;)
         hipLaunchKernelGGL(k_level, dim3(gi), blk, 0, s, ni, level, depth, node_off, d_nodes, order, blo, bhi, nlo,
