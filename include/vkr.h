@@ -155,12 +155,13 @@ enum {
      * with `mirror`, Assets/Tracer.comp:386). */
     VKR_SCENE_TRI_MATERIAL_INDEX = 1u << 0,
     /* Build the triangle subtree on the device (a linear BVH over Morton
-     * codes, one triangle per leaf) instead of the host SAH builder: uploads
-     * of large scenes take milliseconds, traversal is somewhat slower.
-     * max_leaf_size and traversal_cost do not apply to it; when its tree is
-     * deeper than max_depth (0 = 64) the host builder is used instead
-     * (vkr_scene_info.builder says which ran).  Build extension: the
-     * reference has no acceleration structure. */
+     * codes; subtrees of at most max_leaf_size triangles become one leaf)
+     * instead of the host SAH builder: uploads of large scenes take
+     * milliseconds, traversal is a few percent slower, images are the same.
+     * traversal_cost does not apply; when the tree is deeper than max_depth
+     * (0 = 64) the host builder is used instead (vkr_scene_info.builder says
+     * which ran).  Build extension: the reference has no acceleration
+     * structure. */
     VKR_SCENE_DEVICE_BVH = 1u << 1
 };
 
