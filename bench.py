@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--workload", default="soup100k", choices=["soup100k", "batch64"])
     ap.add_argument("--scenes", type=int, default=64, help="batch64: number of scenes")
+    ap.add_argument("--device-bvh", action="store_true", help="trace scenes whose triangle BVH was built on the GPU")
     return ap.parse_args()
 
 
@@ -103,8 +104,15 @@ def bench_batch(a):
     t0 = time.perf_counter()
     wls = scenes.batch_jobs(a.scenes)
     ctx = vkr.Context()
-    sids = [ctx.upload(wl.scene) for wl in wls]
-    log(f"built {len(wls)} scenes in {time.perf_counter() - t0:.1f} s; "
+    hsids = [ctx.upload(wl.scene) for wl in wls]
+    ctx.destroy_scene(ctx.upload(wls[0].scene, device_bvh=True))   # loads the build kernels
+    dsids = [ctx.upload(wl.scene, device_bvh=True) for wl in wls]
+    build_ms = {"host_sah": round(sum(ctx.info(s)["build_ms"] for s in hsids), 2),
+                "device": round(sum(ctx.info(s)["build_ms"] for s in dsids), 2)}
+    for s in (hsids if a.device_bvh else dsids):
+        ctx.destroy_scene(s)
+    sids = dsids if a.device_bvh else hsids
+    log(f"built {len(wls)} scenes in {time.perf_counter() - t0:.1f} s; BVH build ms (sum) {build_ms}; "
         f"tris {sum(len(w.scene.tris) for w in wls)}, pixels*spp {sum(w.width * w.height * w.spp for w in wls)}")
     outs = [torch.zeros((wl.width * wl.height, 4), dtype=torch.uint8, device=dev) for wl in wls]
 
@@ -141,7 +149,8 @@ def bench_batch(a):
         "ms_per_step": round(res["batched_graph"] * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded PCG64 scenes)",
         "config": {"workload": f"batch{len(wls)}", "rays_per_batch": rays, "rays_traversed_per_batch": traced,
-                   "block": a.block, "chunk": a.chunk},
+                   "block": a.block, "chunk": a.chunk, "bvh": "device" if a.device_bvh else "host_sah"},
+        "bvh_build_ms_all_scenes": build_ms,
         "variants_mrays": mr,
         "speedup_batched_vs_plain_per_scene": round(res["plain_per_scene"] / res["batched_graph"], 3),
         "speedup_batched_vs_persistent_per_scene": round(res["persistent_per_scene"] / res["batched_graph"], 3),
@@ -179,9 +188,14 @@ def main():
     wl = scenes.soup100k()
     W, H = wl.width, wl.height
     ctx = vkr.Context()
-    sid = ctx.upload(wl.scene)
+    bvh_build_ms = {}
+    for dev_b in (False, True, True):   # the first device build also loads the build kernels
+        s_ = ctx.upload(wl.scene, device_bvh=dev_b)
+        bvh_build_ms["device" if dev_b else "host_sah"] = round(ctx.info(s_)["build_ms"], 3)
+        ctx.destroy_scene(s_)
+    sid = ctx.upload(wl.scene, device_bvh=a.device_bvh)
     info = ctx.info(sid)
-    log(f"scene: {info}")
+    log(f"scene: {info}; BVH build ms {bvh_build_ms}")
     rgba = torch.zeros((W * H, 4), dtype=torch.uint8, device=dev)
     p = wl.params(sched=a.sched, block=a.block, chunk=a.chunk)
 
@@ -296,7 +310,9 @@ def main():
                    "n_planes": info["n_planes"], "n_emitters": info["n_emitters"],
                    "rays_per_frame": rays, "rays_traversed_per_frame": cnt["rays_traced"],
                    "traversed_mrays": round(cnt["rays_traced"] * a.steps / elapsed / 1e6, 2),
-                   "sched": a.sched, "block": a.block, "chunk": a.chunk},
+                   "sched": a.sched, "block": a.block, "chunk": a.chunk,
+                   "bvh": "device" if a.device_bvh else "host_sah"},
+        "bvh_build_ms": bvh_build_ms,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cpu_scan_baseline": cpu_scan,
