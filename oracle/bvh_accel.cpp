@@ -13,6 +13,7 @@
 // reached through a longer chain of hits each within EPSILON of the previous
 // one is a near-tie the same way as on the GPU.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -28,6 +29,11 @@ struct Accel {
     std::vector<otri> tris;   // scan order, as given
     std::vector<osph> sph;
     std::vector<opln> pln;
+    // Work counts (rays, node visits, triangle / sphere / plane tests), kept
+    // only when counting is on: the GPU's debug counters count the same
+    // events, so the two can be compared ray for ray.
+    bool counting = false;
+    std::atomic<uint64_t> n_rays{0}, n_nodes{0}, n_tris{0}, n_spheres{0}, n_planes{0};
 };
 
 struct F3 { float x, y, z; };
@@ -87,6 +93,21 @@ int trace(const void* ap, const float of[3], const float df[3], float eps, int a
     int nc = 0;
     float tmin = INFINITY;
     bool occluded = false;
+    uint64_t c_nodes = 0, c_tris = 0, c_sph = 0, c_pln = 0;
+    struct Flush {
+        const Accel& A;
+        const uint64_t &nodes, &tris, &sph, &pln;
+        ~Flush()
+        {
+            if (!A.counting) return;
+            Accel& M = const_cast<Accel&>(A);
+            M.n_rays.fetch_add(1, std::memory_order_relaxed);
+            M.n_nodes.fetch_add(nodes, std::memory_order_relaxed);
+            M.n_tris.fetch_add(tris, std::memory_order_relaxed);
+            M.n_spheres.fetch_add(sph, std::memory_order_relaxed);
+            M.n_planes.fetch_add(pln, std::memory_order_relaxed);
+        }
+    } flush{A, c_nodes, c_tris, c_sph, c_pln};
 
     if (!A.bvh.empty && !nan_ray) {
         auto inv = [](float v) { return 1.0f / (std::fabs(v) > 1e-20f ? v : std::copysign(1e-20f, v)); };
@@ -106,6 +127,7 @@ int trace(const void* ap, const float of[3], const float df[3], float eps, int a
         while (true) {
             if ((int32_t)ref >= 0) {
                 const vkr::BvhNode& n = A.bvh.nodes[ref];
+                ++c_nodes;
                 const float cb = cull + std::fabs(cull) * 1e-5f + 1e-5f;
                 float m;
                 std::memcpy(&m, &n.d[2], 4);
@@ -140,10 +162,12 @@ int trace(const void* ap, const float of[3], const float df[3], float eps, int a
                 float t;
                 uint32_t key;
                 if (ref & vkr::kRefSphere) {
+                    ++c_sph;
                     const uint32_t i = A.bvh.sph_key[first + k];
                     t = sphere_t(o, d, P3(A.sph[i].P), A.sph[i].r);
                     key = 0x80000000u | i;
                 } else {
+                    ++c_tris;
                     const uint32_t i = A.bvh.tris[first + k].key;
                     const otri& tr = A.tris[i];
                     t = tri_t(o, d, P3(tr.v0), P3(tr.v1), P3(tr.v2), eps);
@@ -170,6 +194,7 @@ int trace(const void* ap, const float of[3], const float df[3], float eps, int a
     if (any) {
         if (!occluded && !nan_ray) {
             for (const opln& p : A.pln) {
+                ++c_pln;
                 const float t = plane_t(o, d, P3(p.N), p.len);
                 if ((t > eps) && (t < tmax - eps)) { occluded = true; break; }
             }
@@ -189,6 +214,7 @@ int trace(const void* ap, const float of[3], const float df[3], float eps, int a
     if (win_key != 0xFFFFFFFFu) prim = (win_key & 0x80000000u) ? ((1u << 30) | (win_key & 0x7FFFFFFFu)) : win_key;
     if (!nan_ray) {
         for (uint32_t i = 0; i < A.pln.size(); ++i) {
+            ++c_pln;
             const float t = plane_t(o, d, P3(A.pln[i].N), A.pln[i].len);
             if ((t > eps) && (t < best - eps)) {
                 best = t;
@@ -233,5 +259,28 @@ void* vkro_accel_build(const oscene* s)
 void vkro_accel_free(void* a) { delete static_cast<Accel*>(a); }
 
 oaccel_fn vkro_accel_fn(void) { return &trace; }
+
+// Turns the work counts on (and clears them) or off.
+void vkro_accel_count(void* a, int on)
+{
+    Accel* A = static_cast<Accel*>(a);
+    A->counting = on != 0;
+    A->n_rays = 0;
+    A->n_nodes = 0;
+    A->n_tris = 0;
+    A->n_spheres = 0;
+    A->n_planes = 0;
+}
+
+// out = rays, node visits, triangle tests, sphere tests, plane tests.
+void vkro_accel_counts(const void* a, uint64_t out[5])
+{
+    const Accel* A = static_cast<const Accel*>(a);
+    out[0] = A->n_rays;
+    out[1] = A->n_nodes;
+    out[2] = A->n_tris;
+    out[3] = A->n_spheres;
+    out[4] = A->n_planes;
+}
 
 }  // extern "C"

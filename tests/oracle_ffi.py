@@ -112,6 +112,10 @@ def load_accel():
         lib.vkro_accel_free.argtypes = [C.c_void_p]
         lib.vkro_accel_fn.restype = C.c_void_p
         lib.vkro_accel_fn.argtypes = []
+        lib.vkro_accel_count.restype = None
+        lib.vkro_accel_count.argtypes = [C.c_void_p, C.c_int]
+        lib.vkro_accel_counts.restype = None
+        lib.vkro_accel_counts.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
         _accel_lib = lib
     return _accel_lib
 
@@ -146,6 +150,16 @@ class OracleScene:
             self._accel = lib.vkro_accel_build(C.byref(self.c))
             self.c.accel = self._accel
             self.c.accel_fn = lib.vkro_accel_fn()
+
+    def count_work(self, on=True):
+        """Turns the same-BVH trace's work counts on (cleared) or off."""
+        self._accel_lib.vkro_accel_count(self._accel, 1 if on else 0)
+
+    def work(self) -> dict:
+        """Rays, node visits and primitive tests counted since count_work()."""
+        out = (C.c_uint64 * 5)()
+        self._accel_lib.vkro_accel_counts(self._accel, out)
+        return dict(zip(("rays", "nodes_visited", "tris_tested", "spheres_tested", "planes_tested"), out))
 
     def __del__(self):
         if getattr(self, "_accel", None):

@@ -310,6 +310,8 @@ __device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, floa
             if ((int)r.ref < -1) r.leaf<BLOCK, COUNT>(sc, eps, stk, wc);
         }
     }
+    // Lanes without a ray skip the plane list (their result is not read).
+    if (!active) return TravOut{kMiss, tmax, false};
     return r.finish<COUNT>(sc, eps, wc);
 }
 
