@@ -27,6 +27,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(ROOT, "vk-renderer_amd"), os.path.join(ROOT, "tests")]
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 78.6   # non-packed FP32 FMA: 256 CU x 128 FLOP/clk x 2.4 GHz
+# FP32 operations per unit of work (slab test of one box, one Moller-Trumbore
+# triangle test, one sphere test), for the secondary compute figure.
+BOX_FLOP, TRI_FLOP, SPHERE_FLOP = 20, 50, 20
 NODE_B, TRI_B, SPHERE_B, PLANE_B, OUT_B = 64, 48, 20, 16, 4
 
 
@@ -233,6 +237,8 @@ def main():
     algo_bytes = (work["nodes_visited"] * NODE_B + work["tris_tested"] * TRI_B +
                   work["spheres_tested"] * SPHERE_B + work["planes_tested"] * PLANE_B + W * H * OUT_B)
     achieved = algo_bytes / (kern_ms * 1e-3) / 1e9
+    flops = (work["nodes_visited"] * 2 * BOX_FLOP + work["tris_tested"] * TRI_FLOP +
+             work["spheres_tested"] * SPHERE_FLOP)
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_soup100k.json")
     if os.path.exists(tf):
@@ -248,7 +254,11 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "hbm_gbs_measured": round(traffic / (kern_ms * 1e-3) / 1e9, 1) if traffic else None,
                 "algorithmic_bytes_per_launch": int(algo_bytes),
-                "kernel_ms": round(kern_ms, 4)}
+                "kernel_ms": round(kern_ms, 4),
+                # Secondary: traversal and intersection FP32 operations (two
+                # boxes per node visit) against the vector FP32 peak.
+                "fp32_tflops_est": round(flops / (kern_ms * 1e-3) / 1e12, 3),
+                "fp32_peak_tflops": FP32_PEAK_TFLOPS}
 
     variants = {}
     if not a.no_variants:
