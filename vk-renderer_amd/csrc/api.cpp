@@ -33,7 +33,8 @@ namespace {
 //   [root] [n - 1 triangle nodes] [sphere nodes]   (root only with spheres).
 // On success sc->nodes and sc->tris are filled and `sph` holds the sphere
 // records.  Returns false with *err == hipSuccess when the tree is deeper
-// than depth_cap (nothing is kept; the caller builds on the host).
+// than depth_cap or needs 2^26 nodes or more (nothing is kept; the caller
+// builds on the host).
 bool device_build(Scene* sc, uint32_t n, BuildInput bi, uint32_t depth_cap, BuildOutput& sph, uint32_t& n_nodes,
                   uint32_t& n_leaves, uint32_t& depth, hipError_t* err)
 {
@@ -46,10 +47,7 @@ bool device_build(Scene* sc, uint32_t n, BuildInput bi, uint32_t depth_cap, Buil
     const uint32_t node_off = has_sph ? 1u : 0u;
     const uint32_t sph_off = node_off + n - 1u;
     const size_t total = (size_t)sph_off + (has_sph ? sb.nodes.size() : 0u);
-    if (total >= (1u << 26)) {
-        *err = hipErrorInvalidValue;
-        return false;
-    }
+    if (total >= (1u << 26)) return false;   // too many nodes for 32-bit offsets: the host builder may fit
     if ((*err = sc->nodes.alloc(total * sizeof(BvhNode))) != hipSuccess) return false;
     if ((*err = sc->tris.alloc((size_t)n * sizeof(TriRecord))) != hipSuccess) return false;
     LbvhRoot root{};
