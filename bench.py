@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--workload", default="soup100k", choices=["soup100k", "batch64"])
+    ap.add_argument("--workload", default="soup100k", choices=["soup100k", "batch64", "bvhbuild"])
     ap.add_argument("--scenes", type=int, default=64, help="batch64: number of scenes")
     ap.add_argument("--device-bvh", action="store_true", help="trace scenes whose triangle BVH was built on the GPU")
     return ap.parse_args()
@@ -164,17 +164,64 @@ def bench_batch(a):
     ctx.close()
 
 
+def bench_bvh_build(a):
+    """BVH build on upload, host SAH vs device (VKR_SCENE_DEVICE_BVH), on the
+    100k-triangle soup and the 2.09M-triangle heightfield: build ms (median
+    of `steps` uploads after one warm-up upload) and the frame time traced
+    on each tree."""
+    import torch
+
+    import scenes
+    import vkr
+
+    ctx = vkr.Context()
+    res = {}
+    for wl in (scenes.soup100k(), scenes.heightfield()):
+        r = {}
+        for dev_b in (False, True):
+            name = "device" if dev_b else "host_sah"
+            ctx.destroy_scene(ctx.upload(wl.scene, device_bvh=dev_b))   # warm-up (loads the build kernels)
+            ms = []
+            for _ in range(max(1, a.steps)):
+                sid = ctx.upload(wl.scene, device_bvh=dev_b)
+                ms.append(ctx.info(sid)["build_ms"])
+                ctx.destroy_scene(sid)
+            sid = ctx.upload(wl.scene, device_bvh=dev_b)
+            info = ctx.info(sid)
+            rgba = torch.zeros((wl.width * wl.height, 4), dtype=torch.uint8, device="cuda")
+            p = wl.params()
+            ctx.trace(sid, wl.frame, p, rgba8=rgba)
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            for _ in range(3):
+                ctx.trace(sid, wl.frame, p, rgba8=rgba)
+            torch.cuda.synchronize()
+            frame_ms = (time.perf_counter() - ts) / 3 * 1e3
+            ctx.destroy_scene(sid)
+            r[name] = {"build_ms": round(sorted(ms)[len(ms) // 2], 3), "frame_ms": round(frame_ms, 3),
+                       "nodes": info["n_nodes"], "depth": info["depth"], "builder": info["builder"]}
+        r["build_speedup"] = round(r["host_sah"]["build_ms"] / r["device"]["build_ms"], 1)
+        res[wl.name] = r
+        log(f"{wl.name}: {r}")
+    out = {"metric": "BVH build ms on upload (host SAH vs device)", "value": res["heightfield2093058_1920x1080"]
+           ["device"]["build_ms"], "unit": "ms", "n_gpus": 1, "steps": a.steps, "warmup": 1,
+           "higher_is_better": False, "data": "synthetic (seeded PCG64 scenes)",
+           "config": {"workload": "bvhbuild_soup100k_heightfield2m"}, "results": res}
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1",
                           "n_gpus": a.gpus}), flush=True)
         sys.exit(2)
-    if a.workload == "batch64":
+    if a.workload in ("batch64", "bvhbuild"):
         import torch
         if not torch.cuda.is_available():
             raise SystemExit("bench.py needs a GPU")
-        bench_batch(a)
+        (bench_batch if a.workload == "batch64" else bench_bvh_build)(a)
         return
 
     import numpy as np

@@ -35,8 +35,8 @@ namespace {
 // records.  Returns false with *err == hipSuccess when the tree is deeper
 // than depth_cap or needs 2^26 nodes or more (nothing is kept; the caller
 // builds on the host).
-bool device_build(Scene* sc, uint32_t n, BuildInput bi, uint32_t depth_cap, BuildOutput& sph, uint32_t& n_nodes,
-                  uint32_t& n_leaves, uint32_t& depth, hipError_t* err)
+bool device_build(vkr_ctx* ctx, Scene* sc, uint32_t n, BuildInput bi, uint32_t depth_cap, BuildOutput& sph,
+                  uint32_t& n_nodes, uint32_t& n_leaves, uint32_t& depth, hipError_t* err)
 {
     *err = hipSuccess;
     bi.tri_v = nullptr;
@@ -50,10 +50,16 @@ bool device_build(Scene* sc, uint32_t n, BuildInput bi, uint32_t depth_cap, Buil
     if (total >= (1u << 26)) return false;   // too many nodes for 32-bit offsets: the host builder may fit
     if ((*err = sc->nodes.alloc(total * sizeof(BvhNode))) != hipSuccess) return false;
     if ((*err = sc->tris.alloc((size_t)n * sizeof(TriRecord))) != hipSuccess) return false;
+    const size_t need = lbvh_scratch_bytes(n);
+    if (need == 0) {
+        *err = hipErrorInvalidValue;
+        return false;
+    }
+    if (ctx->build_scratch.bytes < need && (*err = ctx->build_scratch.alloc(need)) != hipSuccess) return false;
     LbvhRoot root{};
     const uint32_t tri_cap = has_sph ? depth_cap - 1u : depth_cap;
-    *err = lbvh_build(sc->tris_orig.as<vkr_triangle>(), n, node_off, bi.max_leaf, tri_cap,
-                      sc->nodes.as<BvhNode>(), sc->tris.as<TriRecord>(), &root, nullptr);
+    *err = lbvh_build(sc->tris_orig.as<vkr_triangle>(), n, node_off, bi.max_leaf, tri_cap, ctx->build_scratch.p,
+                      ctx->build_scratch.bytes, sc->nodes.as<BvhNode>(), sc->tris.as<TriRecord>(), &root, nullptr);
     if (*err != hipSuccess) return false;
     const uint32_t dep = has_sph ? 1u + std::max(root.height, sb.depth) : root.height;
     if (dep > depth_cap) {
@@ -176,7 +182,7 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
     if ((desc->flags & VKR_SCENE_DEVICE_BVH) && desc->n_tris >= 2u) {
         hipError_t e = hipSuccess;
         const uint32_t cap = desc->max_depth ? desc->max_depth : 64u;
-        if (device_build(sc.get(), desc->n_tris, bi, cap, bo, n_nodes, n_leaves, depth, &e)) {
+        if (device_build(ctx, sc.get(), desc->n_tris, bi, cap, bo, n_nodes, n_leaves, depth, &e)) {
             builder = VKR_BUILDER_DEVICE_LBVH;
             has_bvh = true;
             t1 = std::chrono::steady_clock::now();

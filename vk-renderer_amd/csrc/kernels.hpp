@@ -147,7 +147,11 @@ struct LbvhRoot {
     float lo[3], hi[3];
     uint32_t height, nodes, leaves;
 };
+// Device scratch a build over n triangles needs (0 when n < 2); the caller
+// passes at least that much to lbvh_build and may reuse it across builds.
+size_t lbvh_scratch_bytes(uint32_t n);
 hipError_t lbvh_build(const vkr_triangle* d_tris, uint32_t n, uint32_t node_off, uint32_t max_leaf,
-                      uint32_t height_cap, BvhNode* d_nodes, TriRecord* d_recs, LbvhRoot* root, hipStream_t s);
+                      uint32_t height_cap, void* scratch, size_t scratch_bytes, BvhNode* d_nodes, TriRecord* d_recs,
+                      LbvhRoot* root, hipStream_t s);
 
 }  // namespace vkr

@@ -394,43 +394,67 @@ __global__ __launch_bounds__(kBuildBlock) void k_records(uint32_t n, const vkr_t
     recs[j] = r;
 }
 
-struct Scratch {
-    void* p = nullptr;
-    ~Scratch()
-    {
-        if (p) (void)hipFree(p);
-    }
-};
-
 inline size_t up256(size_t b) { return (b + 255u) & ~(size_t)255u; }
 
 }  // namespace
 
+namespace {
+
+// Scratch layout of a build over n triangles: 256-byte aligned pieces of one
+// allocation.
+struct ScratchPlan {
+    size_t b_f4n, b_f4i, b_key, b_u32n, b_u32i, b_part, b_part2, b_sort, total;
+};
+
+hipError_t plan_scratch(uint32_t n, ScratchPlan& sp)
+{
+    const uint32_t ni = n - 1u;
+    const uint32_t gn = (n + kBuildBlock - 1u) / kBuildBlock;
+    const uint32_t gi = (ni + kBuildBlock - 1u) / kBuildBlock;
+    size_t sort_bytes = 0;
+    const hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const uint64_t*)nullptr,
+                                                            (uint64_t*)nullptr, (const uint32_t*)nullptr,
+                                                            (uint32_t*)nullptr, (int)n, 0, 64, (hipStream_t)0);
+    if (e != hipSuccess) return e;
+    sp.b_f4n = up256((size_t)n * sizeof(float4));
+    sp.b_f4i = up256((size_t)ni * sizeof(float4));
+    sp.b_key = up256((size_t)n * sizeof(uint64_t));
+    sp.b_u32n = up256((size_t)n * sizeof(uint32_t));
+    sp.b_u32i = up256((size_t)ni * sizeof(uint32_t));
+    sp.b_part = up256((size_t)gn * 6u * sizeof(uint32_t));
+    sp.b_part2 = up256((size_t)gi * 3u * sizeof(uint32_t));
+    sp.b_sort = up256(sort_bytes);
+    sp.total = 2 * sp.b_f4n + 2 * sp.b_f4i + 2 * sp.b_key + 2 * sp.b_u32n + 3 * sp.b_u32i + sp.b_part +
+               sp.b_part2 + 256 + sp.b_sort;
+    return hipSuccess;
+}
+
+}  // namespace
+
+size_t lbvh_scratch_bytes(uint32_t n)
+{
+    ScratchPlan sp{};
+    if (n < 2u || plan_scratch(n, sp) != hipSuccess) return 0;
+    return sp.total;
+}
+
 hipError_t lbvh_build(const vkr_triangle* d_tris, uint32_t n, uint32_t node_off, uint32_t max_leaf,
-                      uint32_t height_cap, BvhNode* d_nodes, TriRecord* d_recs, LbvhRoot* root, hipStream_t s)
+                      uint32_t height_cap, void* scratch, size_t scratch_bytes, BvhNode* d_nodes, TriRecord* d_recs,
+                      LbvhRoot* root, hipStream_t s)
 {
     if (n < 2u || max_leaf < 1u || max_leaf > kMaxLeaf) return hipErrorInvalidValue;
     const uint32_t ni = n - 1u;
     const uint32_t gn = (n + kBuildBlock - 1u) / kBuildBlock;
     const uint32_t gi = (ni + kBuildBlock - 1u) / kBuildBlock;
-    size_t sort_bytes = 0;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const uint64_t*)nullptr,
-                                                      (uint64_t*)nullptr, (const uint32_t*)nullptr,
-                                                      (uint32_t*)nullptr, (int)n, 0, 64, s);
+    ScratchPlan sp{};
+    hipError_t e = plan_scratch(n, sp);
     if (e != hipSuccess) return e;
-    // One scratch allocation, carved into 256-byte aligned pieces.
-    const size_t b_f4n = up256((size_t)n * sizeof(float4));
-    const size_t b_f4i = up256((size_t)ni * sizeof(float4));
-    const size_t b_key = up256((size_t)n * sizeof(uint64_t));
-    const size_t b_u32n = up256((size_t)n * sizeof(uint32_t));
-    const size_t b_u32i = up256((size_t)ni * sizeof(uint32_t));
-    const size_t b_part = up256((size_t)gn * 6u * sizeof(uint32_t));
-    const size_t b_part2 = up256((size_t)gi * 3u * sizeof(uint32_t));
-    const size_t total = 2 * b_f4n + 2 * b_f4i + 2 * b_key + 2 * b_u32n + 3 * b_u32i + b_part + b_part2 + 256 +
-                         up256(sort_bytes);
-    Scratch sc;
-    if ((e = hipMalloc(&sc.p, total)) != hipSuccess) return e;
-    char* q = static_cast<char*>(sc.p);
+    if (!scratch || scratch_bytes < sp.total) return hipErrorInvalidValue;
+    size_t sort_bytes = sp.b_sort;
+    const size_t b_f4n = sp.b_f4n;
+    const size_t b_f4i = sp.b_f4i, b_key = sp.b_key, b_u32n = sp.b_u32n, b_u32i = sp.b_u32i;
+    const size_t b_part = sp.b_part, b_part2 = sp.b_part2;
+    char* q = static_cast<char*>(scratch);
     auto take = [&](size_t b) {
         char* r = q;
         q += b;
@@ -450,7 +474,7 @@ hipError_t lbvh_build(const vkr_triangle* d_tris, uint32_t n, uint32_t node_off,
     uint32_t* part = (uint32_t*)take(b_part);
     uint32_t* part2 = (uint32_t*)take(b_part2);
     uint32_t* small = (uint32_t*)take(256);   // [0..5] centre bounds, [6..8] deepest, live nodes, leaves
-    void* sort_tmp = take(up256(sort_bytes));
+    void* sort_tmp = take(sp.b_sort);
 
     const dim3 blk(kBuildBlock);
     hipLaunchKernelGGL(k_prim_boxes, dim3(gn), blk, 0, s, d_tris, n, blo, bhi, part);
