@@ -89,8 +89,8 @@ int trace(const void* ap, const float of[3], const float df[3], float eps, int a
     const bool nan_ray = std::isnan(o.x) || std::isnan(o.y) || std::isnan(o.z) || std::isnan(d.x) ||
                          std::isnan(d.y) || std::isnan(d.z);
     struct Cand { uint32_t key; float t; };
-    Cand cand[256];
-    int nc = 0;
+    static thread_local std::vector<Cand> cand;   // reused: no allocation once grown
+    cand.clear();
     float tmin = INFINITY;
     bool occluded = false;
     uint64_t c_nodes = 0, c_tris = 0, c_sph = 0, c_pln = 0;
@@ -178,11 +178,11 @@ int trace(const void* ap, const float of[3], const float df[3], float eps, int a
                     if (t < tmax_eps) occluded = true;
                 } else if (t < std::fmin(tmin, tmax_eps) + win) {
                     tmin = std::fmin(tmin, t);
-                    int w = 0;   // drop candidates the smaller tmin ruled out
-                    for (int c = 0; c < nc; ++c)
+                    size_t w = 0;   // drop candidates the smaller tmin ruled out
+                    for (size_t c = 0; c < cand.size(); ++c)
                         if (cand[c].t < tmin + win) cand[w++] = cand[c];
-                    nc = w;
-                    if (nc < 256) cand[nc++] = Cand{key, t};
+                    cand.resize(w);
+                    cand.push_back(Cand{key, t});
                     cull = std::fmin(tmin, tmax_eps) + win;
                 }
             }
@@ -202,10 +202,10 @@ int trace(const void* ap, const float of[3], const float df[3], float eps, int a
         return occluded ? 1 : 0;
     }
     // Replay the scan's rule over the candidates in scan order.
-    std::sort(cand, cand + nc, [](const Cand& a, const Cand& b) { return a.key < b.key; });
+    std::sort(cand.begin(), cand.end(), [](const Cand& a, const Cand& b) { return a.key < b.key; });
     float best = tmax;
     uint32_t win_key = 0xFFFFFFFFu;
-    for (int c = 0; c < nc; ++c)
+    for (size_t c = 0; c < cand.size(); ++c)
         if (cand[c].t < best + eps) {
             best = cand[c].t;
             win_key = cand[c].key;

@@ -107,3 +107,31 @@ def test_accel_nan_ray_misses_like_scan():
                                oracle_ffi.C.byref(prim), oracle_ffi.C.byref(tout))
         out.append((h, prim.value))
     assert out[0] == out[1]
+
+
+def test_accel_many_coincident_candidates():
+    """400 copies of one triangle, all hit at the same t: the scan's
+    `t < best + EPSILON` makes the last copy win, and every copy is a
+    candidate (more than any fixed-size list holds)."""
+    lib = oracle_ffi.load()
+    tri = scenes.tris_from(np.array([[-1, -1, 0]], np.float32), np.array([[1, -1, 0]], np.float32),
+                           np.array([[0, 1, 0]], np.float32))
+    rng = np.random.default_rng(2)
+    other = scenes.random_soup(rng, 200, lo=30.0, hi=50.0)   # off the rays' path
+    scene = vkr.Scene(tris=np.concatenate([other[:100], np.repeat(tri, 400), other[100:]]))
+    fp = oracle_ffi.C.POINTER(oracle_ffi.C.c_float)
+    d = np.array([0.0, 0.0, -1.0], np.float32)   # towards the front face
+    origins = [np.array([rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3), 10.0 + k], np.float32) for k in range(50)]
+    out = []
+    for accel in (False, True):
+        sc = oracle_ffi.OracleScene(scene, accel=accel)
+        res = []
+        for o in origins:
+            prim = oracle_ffi.C.c_uint32(0)
+            tout = oracle_ffi.C.c_float(0)
+            h = lib.vkro_trace_ray(oracle_ffi.C.byref(sc.c), o.ctypes.data_as(fp), d.ctypes.data_as(fp), 1e30, 1e-3,
+                                   oracle_ffi.C.byref(prim), oracle_ffi.C.byref(tout))
+            res.append((h, prim.value, tout.value))
+        out.append(res)
+    assert out[0] == out[1]
+    assert all(h == 1 and p == 100 + 399 for h, p, _ in out[0]), out[0][:3]
