@@ -293,6 +293,14 @@ def main():
             traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
         except (ValueError, OSError):
             traffic = None
+    l2_bytes = None
+    lf = os.path.join(ROOT, "profiles", "l2_soup100k.json")
+    if os.path.exists(lf):
+        try:
+            d = json.load(open(lf))
+            l2_bytes = d["l2_read_bytes_per_launch"] + d["l2_write_bytes_per_launch"]
+        except (ValueError, OSError, KeyError):
+            l2_bytes = None
     # The algorithmic bytes are mostly L2 hits (the node and primitive records
     # of the 100k-triangle frame fit in L2/MALL), so `frac` near 1 does not
     # mean HBM-bound; the measured HBM rate (rocprof traffic over the kernel
@@ -300,6 +308,8 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "hbm_gbs_measured": round(traffic / (kern_ms * 1e-3) / 1e9, 1) if traffic else None,
+                # Bytes the L2 serves per launch (rocprof sector counts) over this run's kernel time.
+                "l2_gbs_measured": round(l2_bytes / (kern_ms * 1e-3) / 1e9, 1) if l2_bytes else None,
                 "algorithmic_bytes_per_launch": int(algo_bytes),
                 "kernel_ms": round(kern_ms, 4),
                 # Secondary: traversal and intersection FP32 operations (two
