@@ -1,4 +1,4 @@
-// bvh_accel.cpp - the CPU baseline's variant (ii): the oracle's path program
+// bvh_accel.cpp - the same-BVH CPU baseline: the oracle's path program
 // tracing the same BVH as the GPU (the build's own host builder,
 // vk-renderer_amd/csrc/bvh.cpp, with the same defaults as vkr_scene_upload).
 //

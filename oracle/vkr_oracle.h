@@ -65,7 +65,7 @@ typedef struct oscene {
     const omat* materials; uint32_t n_materials;
     uint32_t flags;                      /* bit 0: triangles use materials[mat_index] */
     /* Optional (NULL: the literal linear scan).  Set only by the CPU
-     * baseline's variant (ii), which traces the same BVH as the GPU. */
+     * baseline that traces the same BVH as the GPU (bvh_accel.cpp). */
     oaccel_fn accel_fn;
     const void* accel;
 } oscene;
