@@ -130,3 +130,32 @@ def test_adjoint_cost(ctx, torch_cuda):
     ratio = res["adjoint"] / res["forward"]
     print(f"\n[adjoint cost] forward {res['forward'] * 1e3:.2f} ms adjoint {res['adjoint'] * 1e3:.2f} ms ratio {ratio:.2f}")
     assert ratio <= 3.0
+
+
+def test_adjoint_device_tree_matches_host_tree(ctx, torch_cuda):
+    """The adjoint replays paths through the scene's BVH: a triangle subtree
+    built on the device (VKR_SCENE_DEVICE_BVH) gives the same paths, so the
+    same gradient (up to the order of the float additions that reduce it)."""
+    torch = torch_cuda
+    wl0 = scenes.adjoint_scene(width=128, height=128, spp=8)
+    tris = scenes.random_soup(np.random.default_rng(3), 3000, lo=-15.0, hi=15.0)
+    sc = vkr.Scene(tris=tris, spheres=wl0.scene.spheres, planes=wl0.scene.planes)
+    wl = scenes.Workload("adjoint_tris", sc, wl0.frame, 128, 128, 8, wl0.max_depth)
+    theta = scenes.sphere_params(sc.spheres, light_scale=1.0)
+    n = wl.width * wl.height
+    g4 = torch.ones((n, 4), dtype=torch.float32, device="cuda")
+    grads, rads = [], []
+    for dev in (False, True):
+        sid = ctx.upload(sc, device_bvh=dev)
+        assert ctx.info(sid)["builder"] == (vkr.BUILDER_DEVICE_LBVH if dev else vkr.BUILDER_HOST_SAH)
+        rads.append(_forward(ctx, torch, sid, wl, theta))
+        d = torch.zeros(7 * len(sc.spheres) + 1, dtype=torch.float32, device="cuda")
+        ctx.adjoint(sid, wl.frame, wl.params(), g4, d)
+        torch.cuda.synchronize()
+        grads.append(d.double().cpu().numpy())
+        ctx.destroy_scene(sid)
+    assert np.array_equal(rads[0], rads[1])
+    gh, gd = grads
+    agg = np.linalg.norm(gd - gh) / np.linalg.norm(gh)
+    print(f"\n[adjoint host vs device tree] |g| {np.linalg.norm(gh):.4g} aggregate relative difference {agg:.3g}")
+    assert np.linalg.norm(gh) > 0 and agg <= 1e-5
