@@ -98,6 +98,33 @@ def test_batch_mixed_jobs_in_waves(ctx, torch_cuda):
                 assert torch.equal(s[k], b[k]), (block, chunk, k)
 
 
+def test_batch_device_built_scenes(ctx, torch_cuda):
+    """A batch mixing scenes whose triangle trees were built on the device
+    (VKR_SCENE_DEVICE_BVH) with host-built ones: every job's image equals
+    the single-frame trace of its host-built scene."""
+    torch = torch_cuda
+    wls = [scenes.soup100k(seed=90 + i, n_tris=n, width=24 + 8 * (i % 3), height=16 + 8 * (i % 2), spp=2,
+                           max_depth=4, n_spheres=24)
+           for i, n in enumerate((3000, 40000, 500, 20000, 7000, 1200))]
+    host = [ctx.upload(wl.scene) for wl in wls]
+    mixed = [ctx.upload(wl.scene, device_bvh=(i % 2 == 0)) for i, wl in enumerate(wls)]
+    assert [ctx.info(s)["builder"] for s in mixed] == [vkr.BUILDER_DEVICE_LBVH, vkr.BUILDER_HOST_SAH] * 3
+    single = []
+    for wl, sid in zip(wls, host):
+        b = _bufs(torch, wl)
+        ctx.trace(sid, wl.frame, wl.params(), **b)
+        single.append(b)
+    batched = [_bufs(torch, wl) for wl in wls]
+    jobs = [ctx.job(sid, wl.frame, wl.params(), **b) for wl, sid, b in zip(wls, mixed, batched)]
+    ctx.trace_batch(jobs)
+    torch.cuda.synchronize()
+    for i, (s, b) in enumerate(zip(single, batched)):
+        for k in KEYS:
+            assert torch.equal(s[k], b[k]), (i, k)
+    for sid in host + mixed:
+        ctx.destroy_scene(sid)
+
+
 def test_graph_replays_match_eager(ctx, torch_cuda):
     torch = torch_cuda
     wls = _small_jobs()[:3]

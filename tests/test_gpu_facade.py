@@ -83,9 +83,11 @@ def test_pose_replay_matches_oracle(tmp_path):
 
 
 @pytest.mark.gpu
-def test_scene_file_matches_oracle(tmp_path):
+@pytest.mark.parametrize("device_bvh", [False, True], ids=["host_bvh", "device_bvh"])
+def test_scene_file_matches_oracle(tmp_path, device_bvh):
     """--scene traces a scene file (vkr_scene_save) in place of the reference's
-    constant scene; the image matches the oracle on the same arrays."""
+    constant scene, its triangle tree built on the host or (--device-bvh) on
+    the GPU; the image matches the oracle on the same arrays."""
     import torch
     import vkr
     if not torch.cuda.is_available():
@@ -95,7 +97,8 @@ def test_scene_file_matches_oracle(tmp_path):
     path = str(tmp_path / "soup.vkrs")
     wl.scene.save(path)
     prefix = str(tmp_path / "s")
-    r = _run(["--frames", "1", "--res", str(res), "--seeds", "0.5", "--scene", path, "--raw", prefix])
+    r = _run(["--frames", "1", "--res", str(res), "--seeds", "0.5", "--scene", path, "--raw", prefix]
+             + (["--device-bvh"] if device_bvh else []))
     assert r.returncode == 0, r.stdout + r.stderr
     got = np.fromfile(prefix + ".traced0.rgba", np.uint8).reshape(-1, 4)
     sc = vkr.Scene.load(path)

@@ -3,12 +3,14 @@
 // print the frame rate, and write the presented image.
 //
 //   vkr_headless [--frames N] [--res R] [--present WxH] [--seeds a,b,c]
-//                [--poses file] [--scene file.vkrs] [--out image.ppm] [--raw prefix]
+//                [--poses file] [--scene file.vkrs [--device-bvh]] [--out image.ppm]
+//                [--raw prefix]
 // --poses replays recorded camera input in place of Main.cpp:165-190's
 // keyboard/mouse handling: one "x y z pitch yaw" line per frame (cycled;
 // '#' starts a comment), each applied through Camera::update.
 // --scene traces a scene file (vkr_scene_save) instead of the reference's
-// constant scene.
+// constant scene; --device-bvh builds its triangle tree on the GPU
+// (VKR_SCENE_DEVICE_BVH).
 // --raw writes prefix.traced0.rgba, prefix.traced1.rgba, prefix.present.rgba
 // and prefix.seeds.txt for checking against the CPU restatement.
 #include <chrono>
@@ -56,6 +58,7 @@ int main(int argc, char** argv)
     unsigned res = 1024, pw = 1024, ph = 768;
     std::vector<float> seeds;
     std::string out, raw, poses_path, scene_path;
+    bool device_bvh = false;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         auto next = [&]() { return i + 1 < argc ? std::string(argv[++i]) : std::string(); };
@@ -66,6 +69,7 @@ int main(int argc, char** argv)
         else if (a == "--raw") raw = next();
         else if (a == "--poses") poses_path = next();
         else if (a == "--scene") scene_path = next();
+        else if (a == "--device-bvh") device_bvh = true;
         else if (a == "--seeds") {
             const std::string s = next();
             size_t p = 0;
@@ -104,6 +108,7 @@ int main(int argc, char** argv)
             std::printf("{\"error\": %d, \"message\": \"%s\"}\n", 4, vkr_last_error_string());
             return 2;
         }
+        if (device_bvh) desc.flags |= VKR_SCENE_DEVICE_BVH;
         const auto e = device.SetScene(desc);
         vkr_scene_free(&desc);
         if (e != GraphicsDevice::Error::SUCCESS) {
