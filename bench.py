@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--workload", default="soup100k", choices=["soup100k", "batch64", "bvhbuild"])
     ap.add_argument("--scenes", type=int, default=64, help="batch64: number of scenes")
     ap.add_argument("--device-bvh", action="store_true", help="trace scenes whose triangle BVH was built on the GPU")
+    ap.add_argument("--tris", type=int, default=100_000, help="soup workload: number of triangles (the headline is 100000)")
     return ap.parse_args()
 
 
@@ -236,7 +237,7 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
 
-    wl = scenes.soup100k()
+    wl = scenes.soup100k(n_tris=a.tris)
     W, H = wl.width, wl.height
     ctx = vkr.Context()
     bvh_build_ms = {}
@@ -286,16 +287,19 @@ def main():
     achieved = algo_bytes / (kern_ms * 1e-3) / 1e9
     flops = (work["nodes_visited"] * 2 * BOX_FLOP + work["tris_tested"] * TRI_FLOP +
              work["spheres_tested"] * SPHERE_FLOP)
+    # The rocprof traffic files describe the 100k-triangle frame on the host
+    # builder's tree only.
+    measured = a.tris == 100_000 and not a.device_bvh
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_soup100k.json")
-    if os.path.exists(tf):
+    if measured and os.path.exists(tf):
         try:
             traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
         except (ValueError, OSError):
             traffic = None
     l2_bytes = None
     lf = os.path.join(ROOT, "profiles", "l2_soup100k.json")
-    if os.path.exists(lf):
+    if measured and os.path.exists(lf):
         try:
             d = json.load(open(lf))
             l2_bytes = d["l2_read_bytes_per_launch"] + d["l2_write_bytes_per_launch"]
@@ -372,7 +376,8 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (seeded PCG64 scene, FrameData.seed 0)",
-        "config": {"workload": "soup100k_1920x1080_4spp_4bounce", "width": W, "height": H, "spp": wl.spp,
+        "config": {"workload": f"soup{wl.scene.tris.shape[0] // 1000}k_1920x1080_4spp_4bounce", "width": W, "height": H,
+                   "spp": wl.spp,
                    "max_depth": wl.max_depth, "n_tris": info["n_tris"], "n_spheres": info["n_spheres"],
                    "n_planes": info["n_planes"], "n_emitters": info["n_emitters"],
                    "rays_per_frame": rays, "rays_traversed_per_frame": cnt["rays_traced"],
