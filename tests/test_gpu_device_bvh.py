@@ -84,6 +84,31 @@ def _two_tris():
     return scenes.Workload("two_tris", sc, scenes.frame(cam, 1.0), 64, 64, 2, 3)
 
 
+def _coincident():
+    """Four copies of one triangle: every centre is the same point, so the
+    centre bounds are empty, every Morton key is equal and the tree splits by
+    index alone.  The scan's `t < best + EPSILON` makes the last copy win every
+    hit (four copies: the kernel keeps up to four candidates within 4 EPSILON
+    of the nearest hit, more are a documented near-tie)."""
+    M = scenes.reference_materials()
+    one = scenes.tris_from(np.array([[-4, -4, 0]], np.float32), np.array([[-4, 4, 0]], np.float32),
+                           np.array([[4, -4, 0]], np.float32))
+    sph = scenes.spheres_from([M["light"], M["matte_white"]], [(0.0, 12.0, -8.0), (6.0, 6.0, -3.0)], [4.0, 1.5])
+    cam = vkr.camera_from_angles((0.0, 0.0, -10.0), 0.0, 90.0)
+    sc = vkr.Scene(tris=np.repeat(one, 4), spheres=sph)
+    return scenes.Workload("coincident", sc, scenes.frame(cam, 1.0), 48, 48, 2, 3)
+
+
+def _all_non_finite():
+    """Only non-finite triangles (never hit) beside the spheres64 scene: the
+    whole triangle subtree gets boxes no ray enters."""
+    w = scenes.spheres64()
+    tris = scenes.random_soup(np.random.default_rng(4), 100, lo=-5.0, hi=5.0)
+    tris["v1"][:, 0] = np.nan
+    sc = vkr.Scene(tris=tris, spheres=w.scene.spheres, planes=w.scene.planes)
+    return scenes.Workload("all_non_finite", sc, w.frame, 128, 128, w.spp, w.max_depth)
+
+
 CASES = {
     "soup3000": _soup,
     "many_emitters": scenes.many_emitters,
@@ -91,6 +116,8 @@ CASES = {
     "tris_only": _tris_only,
     "non_finite": _non_finite,
     "two_tris": _two_tris,
+    "coincident": _coincident,
+    "all_non_finite": _all_non_finite,
 }
 
 
