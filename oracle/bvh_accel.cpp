@@ -9,9 +9,10 @@
 // EPSILON` over triangles then spheres in scan order, then planes with
 // `t < best - EPSILON`.  The smallest hit t* is always accepted by the scan,
 // so only hits below t* + 4 EPSILON can still change the winner: they are all
-// kept (no cap) and replayed in scan order after the traversal.  A winner
-// reached through a longer chain of hits each within EPSILON of the previous
-// one is a near-tie the same way as on the GPU.
+// kept (no cap) and replayed in scan order after the traversal.  When the
+// replayed chain ends within EPSILON of that window's edge (a longer chain of
+// hits each within EPSILON of the previous one), the window is doubled and
+// the ray traced again, so the result is always the scan's.
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -85,14 +86,15 @@ int trace(const void* ap, const float of[3], const float df[3], float eps, int a
     const F3 o = P3(of), d = P3(df);
     const float tmax = *t_io;
     const float tmax_eps = tmax + eps;
-    const float win = 4.0f * eps;
+    float win = 4.0f * eps;   // widened when a chain of acceptances reaches its edge
     const bool nan_ray = std::isnan(o.x) || std::isnan(o.y) || std::isnan(o.z) || std::isnan(d.x) ||
                          std::isnan(d.y) || std::isnan(d.z);
     struct Cand { uint32_t key; float t; };
     static thread_local std::vector<Cand> cand;   // reused: no allocation once grown
-    cand.clear();
     float tmin = INFINITY;
     bool occluded = false;
+    float best = tmax;
+    uint32_t win_key = 0xFFFFFFFFu;
     uint64_t c_nodes = 0, c_tris = 0, c_sph = 0, c_pln = 0;
     struct Flush {
         const Accel& A;
@@ -109,6 +111,9 @@ int trace(const void* ap, const float of[3], const float df[3], float eps, int a
         }
     } flush{A, c_nodes, c_tris, c_sph, c_pln};
 
+  for (;;) {
+    cand.clear();
+    tmin = INFINITY;
     if (!A.bvh.empty && !nan_ray) {
         auto inv = [](float v) { return 1.0f / (std::fabs(v) > 1e-20f ? v : std::copysign(1e-20f, v)); };
         const float ix = inv(d.x), iy = inv(d.y), iz = inv(d.z);
@@ -191,6 +196,21 @@ int trace(const void* ap, const float of[3], const float df[3], float eps, int a
             ref = stack[--sp];
         }
     }
+    if (any) break;
+    // Replay the scan's rule over the candidates in scan order.
+    std::sort(cand.begin(), cand.end(), [](const Cand& a, const Cand& b) { return a.key < b.key; });
+    best = tmax;
+    win_key = 0xFFFFFFFFu;
+    for (size_t c = 0; c < cand.size(); ++c)
+        if (cand[c].t < best + eps) {
+            best = cand[c].t;
+            win_key = cand[c].key;
+        }
+    // Exact unless the chain ended within EPSILON of the window's edge,
+    // where a hit outside the window could still be accepted: widen, retrace.
+    if (win_key == 0xFFFFFFFFu || best < tmin + win - eps) break;
+    win *= 2.0f;
+  }
     if (any) {
         if (!occluded && !nan_ray) {
             for (const opln& p : A.pln) {
@@ -201,15 +221,6 @@ int trace(const void* ap, const float of[3], const float df[3], float eps, int a
         }
         return occluded ? 1 : 0;
     }
-    // Replay the scan's rule over the candidates in scan order.
-    std::sort(cand.begin(), cand.end(), [](const Cand& a, const Cand& b) { return a.key < b.key; });
-    float best = tmax;
-    uint32_t win_key = 0xFFFFFFFFu;
-    for (size_t c = 0; c < cand.size(); ++c)
-        if (cand[c].t < best + eps) {
-            best = cand[c].t;
-            win_key = cand[c].key;
-        }
     uint32_t prim = 0xFFFFFFFFu;
     if (win_key != 0xFFFFFFFFu) prim = (win_key & 0x80000000u) ? ((1u << 30) | (win_key & 0x7FFFFFFFu)) : win_key;
     if (!nan_ray) {

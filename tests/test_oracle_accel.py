@@ -162,3 +162,37 @@ def test_accel_heightfield_sample_equals_scan():
     assert int(ref["ray_count"].sum()) > 0
     for k in OUT_KEYS:
         assert np.array_equal(ref[k], acc[k]), k
+
+
+def test_accel_chain_beyond_the_window():
+    """Ten parallel copies of a triangle 0.9 EPSILON apart along the ray, in
+    scan order from the nearest: the scan accepts each in turn (t < best +
+    EPSILON) and ends 8.1 EPSILON past the nearest hit, beyond the 4 EPSILON
+    window the replay starts with; the window is widened until the chain ends
+    inside it."""
+    lib = oracle_ffi.load()
+    k = np.arange(10, dtype=np.float32)
+    z = (-k * np.float32(0.9e-3)).astype(np.float32)
+    v0 = np.stack([np.full(10, -2.0), np.full(10, -2.0), z], axis=1).astype(np.float32)
+    v1 = np.stack([np.full(10, 2.0), np.full(10, -2.0), z], axis=1).astype(np.float32)
+    v2 = np.stack([np.full(10, 0.0), np.full(10, 2.0), z], axis=1).astype(np.float32)
+    # Wound to face a ray going down -z.
+    tris = scenes.tris_from(v0, v1, v2)
+    other = scenes.random_soup(np.random.default_rng(6), 100, lo=30.0, hi=50.0)
+    scene = vkr.Scene(tris=np.concatenate([tris, other]))
+    fp = oracle_ffi.C.POINTER(oracle_ffi.C.c_float)
+    d = np.array([0.0, 0.0, -1.0], np.float32)
+    out = []
+    for accel in (False, True):
+        sc = oracle_ffi.OracleScene(scene, accel=accel)
+        res = []
+        for x in (-0.3, 0.0, 0.2):
+            o = np.array([x, 0.1, 10.0], np.float32)
+            prim = oracle_ffi.C.c_uint32(0)
+            tout = oracle_ffi.C.c_float(0)
+            h = lib.vkro_trace_ray(oracle_ffi.C.byref(sc.c), o.ctypes.data_as(fp), d.ctypes.data_as(fp), 1e30, 1e-3,
+                                   oracle_ffi.C.byref(prim), oracle_ffi.C.byref(tout))
+            res.append((h, prim.value, tout.value))
+        out.append(res)
+    assert all(h == 1 and p == 9 for h, p, _ in out[0]), out[0]
+    assert out[0] == out[1]
