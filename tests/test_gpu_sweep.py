@@ -1,5 +1,6 @@
-"""GPU parity over a fixed sweep of small random scenes: each seed draws its
-own mix of sphere materials (diffuse, metal, dielectric, emissive, with random
+"""GPU parity over a fixed sweep of small random scenes, with the triangle
+tree built on the host and on the device: each seed draws its own mix of
+sphere materials (diffuse, metal, dielectric, emissive, with random
 roughness and metalness), planes, a triangle soup with per-triangle materials
 or the reference's single mirror material, an image size (odd sizes
 included), samples per pixel, bounce depth and epsilon.  Every pixel must
@@ -70,12 +71,15 @@ def _scene(seed):
     return scenes.Workload(f"sweep{seed}", sc, fr, w, h, spp, depth), eps
 
 
+@pytest.mark.parametrize("device_bvh", [False, True], ids=["host_bvh", "device_bvh"])
 @pytest.mark.parametrize("seed", range(48))
-def test_random_scene_matches_oracle(ctx, torch_cuda, seed):
+def test_random_scene_matches_oracle(ctx, torch_cuda, seed, device_bvh):
     wl, eps = _scene(seed)
     print(f"\n[sweep {seed}] {len(wl.scene.tris)} tris {len(wl.scene.spheres)} spheres {len(wl.scene.planes)} planes "
           f"{wl.width}x{wl.height} spp {wl.spp} depth {wl.max_depth} eps {eps} flags {wl.scene.flags}", flush=True)
-    sid = ctx.upload(wl.scene)
+    sid = ctx.upload(wl.scene, device_bvh=device_bvh)
+    if device_bvh and len(wl.scene.tris) >= 2:
+        assert ctx.info(sid)["builder"] == vkr.BUILDER_DEVICE_LBVH
     g = gpu_trace_eps(ctx, sid, wl, eps)
     r = oracle_ffi.trace(wl.scene, wl.frame, wl.width, wl.height, wl.spp, wl.max_depth, epsilon=eps)
     st = compare(g, r)
