@@ -159,3 +159,32 @@ def test_adjoint_device_tree_matches_host_tree(ctx, torch_cuda):
     agg = np.linalg.norm(gd - gh) / np.linalg.norm(gh)
     print(f"\n[adjoint host vs device tree] |g| {np.linalg.norm(gh):.4g} aggregate relative difference {agg:.3g}")
     assert np.linalg.norm(gh) > 0 and agg <= 1e-5
+
+
+def test_adjoint_is_linear_in_the_output_gradient(ctx, torch_cuda):
+    """At the full adjoint workload (256^2, 16 spp, 4 bounces): the parameter
+    gradient is linear in the image gradient, adjoint(2 g1) = 2 adjoint(g1)
+    and adjoint(g1 + g2) = adjoint(g1) + adjoint(g2), up to the order of the
+    float additions that reduce it."""
+    torch = torch_cuda
+    wl = scenes.adjoint_scene()
+    sid = ctx.upload(wl.scene)
+    n = wl.width * wl.height
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    g1 = torch.rand((n, 4), device="cuda", generator=gen) - 0.5
+    g2 = torch.rand((n, 4), device="cuda", generator=gen) - 0.5
+
+    def adj(g):
+        d = torch.zeros(7 * len(wl.scene.spheres) + 1, dtype=torch.float32, device="cuda")
+        ctx.adjoint(sid, wl.frame, wl.params(), g.contiguous(), d)
+        torch.cuda.synchronize()
+        return d.double().cpu().numpy()
+
+    a1, a2, a12, a1x2 = adj(g1), adj(g2), adj(g1 + g2), adj(2.0 * g1)
+    scale = np.linalg.norm(a1) + np.linalg.norm(a2)
+    e_scale = np.linalg.norm(a1x2 - 2.0 * a1) / (2.0 * np.linalg.norm(a1))
+    e_sum = np.linalg.norm(a12 - a1 - a2) / scale
+    print(f"\n[adjoint linearity] |a1| {np.linalg.norm(a1):.4g} scale err {e_scale:.3g} sum err {e_sum:.3g}")
+    assert np.linalg.norm(a1) > 0
+    assert e_scale <= 1e-6 and e_sum <= 1e-5
+    ctx.destroy_scene(sid)
