@@ -162,3 +162,30 @@ def test_random_scenes_in_one_batch(ctx, torch_cuda):
             assert torch.equal(x, y), (i, k)
     for sid in sids:
         ctx.destroy_scene(sid)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_filter_random_sizes_match_oracle(ctx, torch_cuda, seed):
+    """Fullscreen.frag as a kernel (bilinear fetch with a black border, the
+    temporal-variance gate between the current and the previous traced image,
+    v flip) on random traced and present sizes, up- and down-scaling, with
+    partly changed previous images: bit-exact against the oracle."""
+    torch = torch_cuda
+    rng = np.random.default_rng(3000 + seed)
+    sw, sh = int(rng.integers(1, 300)), int(rng.integers(1, 300))
+    dw, dh = int(rng.integers(1, 400)), int(rng.integers(1, 400))
+    cur = rng.integers(0, 256, (sw * sh, 4), dtype=np.uint8)
+    prev = cur.copy()
+    changed = rng.random(sw * sh) < rng.uniform(0.0, 1.0)
+    prev[changed] = np.clip(prev[changed].astype(np.int32) + rng.integers(-3, 4, (int(changed.sum()), 4)), 0,
+                            255).astype(np.uint8)
+    dev = torch.device("cuda:0")
+    tc, tp = torch.from_numpy(cur).to(dev), torch.from_numpy(prev).to(dev)
+    to = torch.zeros((dw * dh, 4), dtype=torch.uint8, device=dev)
+    ctx.filter(tc, tp, sw, sh, to, dw, dh)
+    torch.cuda.synchronize()
+    ref = np.zeros((dw * dh, 4), np.uint8)
+    assert oracle_ffi.load().vkro_filter(cur.ctypes.data, prev.ctypes.data, sw, sh, ref.ctypes.data, dw, dh) == 0
+    got = to.cpu().numpy()
+    print(f"\n[filter sweep {seed}] {sw}x{sh} -> {dw}x{dh}: {(got == ref).all(axis=1).mean():.6f} equal", flush=True)
+    assert np.array_equal(got, ref)
