@@ -71,16 +71,22 @@ def _scene(seed):
     return scenes.Workload(f"sweep{seed}", sc, fr, w, h, spp, depth), eps
 
 
+@pytest.mark.parametrize("variant", ["auto", "other"], ids=["kernel_auto", "kernel_other"])
 @pytest.mark.parametrize("device_bvh", [False, True], ids=["host_bvh", "device_bvh"])
 @pytest.mark.parametrize("seed", range(48))
-def test_random_scene_matches_oracle(ctx, torch_cuda, seed, device_bvh):
+def test_random_scene_matches_oracle(ctx, torch_cuda, seed, device_bvh, variant):
+    """variant "other" forces the kernel variant the scene would not get:
+    the wave-shared ray pool below two emitters, single rays from two on."""
     wl, eps = _scene(seed)
     print(f"\n[sweep {seed}] {len(wl.scene.tris)} tris {len(wl.scene.spheres)} spheres {len(wl.scene.planes)} planes "
           f"{wl.width}x{wl.height} spp {wl.spp} depth {wl.max_depth} eps {eps} flags {wl.scene.flags}", flush=True)
     sid = ctx.upload(wl.scene, device_bvh=device_bvh)
     if device_bvh and len(wl.scene.tris) >= 2:
         assert ctx.info(sid)["builder"] == vkr.BUILDER_DEVICE_LBVH
-    g = gpu_trace_eps(ctx, sid, wl, eps)
+    flags = 0
+    if variant == "other":
+        flags = vkr.TRACE_RAY_POOL if ctx.info(sid)["n_emitters"] < 2 else vkr.TRACE_NO_RAY_POOL
+    g = gpu_trace_eps(ctx, sid, wl, eps, flags)
     r = oracle_ffi.trace(wl.scene, wl.frame, wl.width, wl.height, wl.spp, wl.max_depth, epsilon=eps)
     st = compare(g, r)
     print(f"[sweep {seed}] " + " ".join(f"{k}={v}" for k, v in st.items()), flush=True)
@@ -89,7 +95,7 @@ def test_random_scene_matches_oracle(ctx, torch_cuda, seed, device_bvh):
     ctx.destroy_scene(sid)
 
 
-def gpu_trace_eps(ctx, sid, wl, eps):
+def gpu_trace_eps(ctx, sid, wl, eps, flags=0):
     """gpu_trace with the workload's epsilon set in the trace parameters."""
     orig = wl.params
 
@@ -100,7 +106,7 @@ def gpu_trace_eps(ctx, sid, wl, eps):
 
     wl2 = scenes.Workload(wl.name, wl.scene, wl.frame, wl.width, wl.height, wl.spp, wl.max_depth)
     object.__setattr__(wl2, "params", params)
-    return gpu_trace(ctx, sid, wl2)
+    return gpu_trace(ctx, sid, wl2, flags=flags)
 
 
 @pytest.mark.parametrize("seed", range(12))
