@@ -435,6 +435,9 @@ VKR_STATIC_ASSERT(sizeof(vkr_frame_data) == 96, "FrameData is 96 bytes");
 VKR_STATIC_ASSERT(offsetof(vkr_frame_data, seed) == 4, "FrameData.seed @4");
 VKR_STATIC_ASSERT(offsetof(vkr_frame_data, light_pos) == 16, "FrameData.light_pos @16");
 VKR_STATIC_ASSERT(offsetof(vkr_frame_data, camera) == 32, "FrameData.camera @32");
+VKR_STATIC_ASSERT(sizeof(vkr_scene_info) == 56, "vkr_scene_info is 56 bytes");
+VKR_STATIC_ASSERT(offsetof(vkr_scene_info, device_bytes) == 32, "vkr_scene_info.device_bytes @32");
+VKR_STATIC_ASSERT(offsetof(vkr_scene_info, builder) == 48, "vkr_scene_info.builder @48");
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -69,6 +69,14 @@ def test_layouts_are_std430():
     assert vkr.FRAME.fields["light_pos"][1] == 16 and vkr.FRAME.fields["camera"][1] == 32
 
 
+def test_scene_info_mirror_matches_the_header():
+    """The ctypes mirror of vkr_scene_info (include/vkr.h asserts the same)."""
+    import ctypes as C
+    assert C.sizeof(vkr.SceneInfo) == 56
+    assert vkr.SceneInfo.device_bytes.offset == 32 and vkr.SceneInfo.build_ms.offset == 40
+    assert vkr.SceneInfo.builder.offset == 48
+
+
 def test_scene_generators_are_deterministic():
     a = scenes.soup100k(seed=3, n_tris=2000)
     b = scenes.soup100k(seed=3, n_tris=2000)
