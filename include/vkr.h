@@ -383,7 +383,7 @@ vkr_error vkr_filter_frame(vkr_ctx* ctx, const uint8_t* current, const uint8_t* 
                            uint32_t src_w, uint32_t src_h, uint8_t* out,
                            uint32_t dst_w, uint32_t dst_h, void* stream);
 
-/* Camera::update (Source/Camera.cpp:91-107): orthonormal basis from pitch/yaw
+/* Camera::update (Source/Camera.cpp:48-64): orthonormal basis from pitch/yaw
  * in degrees (pitch clamped to [-89, 89]).  pos is copied through.  Host
  * only; needs no context or GPU. */
 vkr_error vkr_camera_from_angles(const float pos[3], float pitch, float yaw, vkr_camera* out);

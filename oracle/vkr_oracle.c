@@ -897,7 +897,7 @@ int vkro_filter(const uint8_t* cur, const uint8_t* prev, uint32_t sw, uint32_t s
 }
 
 /* ------------------------------------------------------------------------ */
-/* Camera::update, Source/Camera.cpp:91-107 (GLM float semantics:             */
+/* Camera::update, Source/Camera.cpp:48-64 (GLM float semantics:             */
 /* radians = deg * float(pi/180); normalize = v * (1 / sqrt(dot(v, v)));      */
 /* dot = (x*x' + y*y') + z*z'; std::cos/sin on float = cosf/sinf)             */
 /* ------------------------------------------------------------------------ */

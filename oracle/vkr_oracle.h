@@ -124,7 +124,7 @@ void  vkro_nee_contribution(const float P[3], const float N[3], const float cam_
 int vkro_filter(const uint8_t* cur, const uint8_t* prev, uint32_t src_w, uint32_t src_h,
                 uint8_t* out, uint32_t dst_w, uint32_t dst_h);
 
-/* Camera::update, Source/Camera.cpp:91-107 (pitch clamped to [-89, 89]). */
+/* Camera::update, Source/Camera.cpp:48-64 (pitch clamped to [-89, 89]). */
 void vkro_camera_update(const float pos[3], float pitch, float yaw, ocam* out);
 
 #ifdef __cplusplus

@@ -45,55 +45,65 @@ def build():
     subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
 
 
+def _configure(lib):
+    """Declares the argument and result types of the oracle's exports."""
+    fp = C.POINTER(C.c_float)
+    lib.vkro_trace_pixels.restype = C.c_int
+    lib.vkro_trace_pixels.argtypes = [C.POINTER(OScene), C.c_void_p, C.POINTER(OParams), C.c_void_p, C.c_uint32,
+                                      C.POINTER(OOut), C.c_int]
+    for n in ("vkro_cos", "vkro_sin"):
+        getattr(lib, n).restype = C.c_float
+        getattr(lib, n).argtypes = [C.c_float]
+    lib.vkro_pow.restype = C.c_float
+    lib.vkro_pow.argtypes = [C.c_float, C.c_float]
+    lib.vkro_rand_sequence.restype = None
+    lib.vkro_rand_sequence.argtypes = [C.c_float, C.c_float, C.c_float, C.c_uint32, fp]
+    lib.vkro_tri_intersect.restype = C.c_float
+    lib.vkro_tri_intersect.argtypes = [fp, fp, C.c_void_p, C.c_float]
+    lib.vkro_sphere_intersect.restype = C.c_float
+    lib.vkro_sphere_intersect.argtypes = [fp, fp, C.c_void_p]
+    lib.vkro_plane_intersect.restype = C.c_float
+    lib.vkro_plane_intersect.argtypes = [fp, fp, C.c_void_p]
+    lib.vkro_trace_ray.restype = C.c_int
+    lib.vkro_trace_ray.argtypes = [C.POINTER(OScene), fp, fp, C.c_float, C.c_float, C.POINTER(C.c_uint32), fp]
+    lib.vkro_distribution_ggx.restype = C.c_float
+    lib.vkro_distribution_ggx.argtypes = [fp, fp, C.c_float]
+    lib.vkro_geometry_smith.restype = C.c_float
+    lib.vkro_geometry_smith.argtypes = [fp, fp, fp, C.c_float]
+    lib.vkro_fresnel_schlick.restype = None
+    lib.vkro_fresnel_schlick.argtypes = [C.c_float, fp, fp]
+    lib.vkro_schlick.restype = C.c_float
+    lib.vkro_schlick.argtypes = [C.c_float, C.c_float]
+    lib.vkro_jitter.restype = None
+    lib.vkro_jitter.argtypes = [fp, C.c_float, C.c_float, C.c_float, fp]
+    lib.vkro_reflect.restype = None
+    lib.vkro_reflect.argtypes = [fp, fp, fp]
+    lib.vkro_refract.restype = None
+    lib.vkro_refract.argtypes = [fp, fp, C.c_float, fp]
+    lib.vkro_primary_dir.restype = None
+    lib.vkro_primary_dir.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, fp]
+    lib.vkro_filter.restype = C.c_int
+    lib.vkro_filter.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32]
+    lib.vkro_nee_contribution.restype = None
+    lib.vkro_nee_contribution.argtypes = [fp, fp, fp, C.c_void_p, C.c_void_p, fp, fp]
+    lib.vkro_camera_update.restype = None
+    lib.vkro_camera_update.argtypes = [fp, C.c_float, C.c_float, C.c_void_p]
+    return lib
+
+
 def load():
     global _lib
     if _lib is None:
         if not os.path.exists(ORACLE_LIB):
             build()
-        lib = C.CDLL(ORACLE_LIB)
-        fp = C.POINTER(C.c_float)
-        lib.vkro_trace_pixels.restype = C.c_int
-        lib.vkro_trace_pixels.argtypes = [C.POINTER(OScene), C.c_void_p, C.POINTER(OParams), C.c_void_p, C.c_uint32,
-                                          C.POINTER(OOut), C.c_int]
-        for n in ("vkro_cos", "vkro_sin"):
-            getattr(lib, n).restype = C.c_float
-            getattr(lib, n).argtypes = [C.c_float]
-        lib.vkro_pow.restype = C.c_float
-        lib.vkro_pow.argtypes = [C.c_float, C.c_float]
-        lib.vkro_rand_sequence.restype = None
-        lib.vkro_rand_sequence.argtypes = [C.c_float, C.c_float, C.c_float, C.c_uint32, fp]
-        lib.vkro_tri_intersect.restype = C.c_float
-        lib.vkro_tri_intersect.argtypes = [fp, fp, C.c_void_p, C.c_float]
-        lib.vkro_sphere_intersect.restype = C.c_float
-        lib.vkro_sphere_intersect.argtypes = [fp, fp, C.c_void_p]
-        lib.vkro_plane_intersect.restype = C.c_float
-        lib.vkro_plane_intersect.argtypes = [fp, fp, C.c_void_p]
-        lib.vkro_trace_ray.restype = C.c_int
-        lib.vkro_trace_ray.argtypes = [C.POINTER(OScene), fp, fp, C.c_float, C.c_float, C.POINTER(C.c_uint32), fp]
-        lib.vkro_distribution_ggx.restype = C.c_float
-        lib.vkro_distribution_ggx.argtypes = [fp, fp, C.c_float]
-        lib.vkro_geometry_smith.restype = C.c_float
-        lib.vkro_geometry_smith.argtypes = [fp, fp, fp, C.c_float]
-        lib.vkro_fresnel_schlick.restype = None
-        lib.vkro_fresnel_schlick.argtypes = [C.c_float, fp, fp]
-        lib.vkro_schlick.restype = C.c_float
-        lib.vkro_schlick.argtypes = [C.c_float, C.c_float]
-        lib.vkro_jitter.restype = None
-        lib.vkro_jitter.argtypes = [fp, C.c_float, C.c_float, C.c_float, fp]
-        lib.vkro_reflect.restype = None
-        lib.vkro_reflect.argtypes = [fp, fp, fp]
-        lib.vkro_refract.restype = None
-        lib.vkro_refract.argtypes = [fp, fp, C.c_float, fp]
-        lib.vkro_primary_dir.restype = None
-        lib.vkro_primary_dir.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, fp]
-        lib.vkro_filter.restype = C.c_int
-        lib.vkro_filter.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32]
-        lib.vkro_nee_contribution.restype = None
-        lib.vkro_nee_contribution.argtypes = [fp, fp, fp, C.c_void_p, C.c_void_p, fp, fp]
-        lib.vkro_camera_update.restype = None
-        lib.vkro_camera_update.argtypes = [fp, C.c_float, C.c_float, C.c_void_p]
-        _lib = lib
+        _lib = _configure(C.CDLL(ORACLE_LIB))
     return _lib
+
+
+def load_from(path: str):
+    """Loads another build of the oracle source (the mutation tests compile
+    copies of vkr_oracle.c with one planted mistake each)."""
+    return _configure(C.CDLL(path))
 
 
 _accel_lib = None
@@ -168,14 +178,15 @@ class OracleScene:
 
 
 def trace(scene, frame, width, height, spp=1, max_depth=1, epsilon=1e-3, kernel=KERNEL_PATH, flags=0,
-          pixels=None, threads=None, decisions=None, accel=False, oscene=None):
+          pixels=None, threads=None, decisions=None, accel=False, oscene=None, lib=None):
     """Runs the oracle over `pixels` (flat y*W+x indices; None = full frame).
 
     accel=True traces the product's BVH instead of the linear scan (same
-    results); oscene reuses an OracleScene built before.
+    results); oscene reuses an OracleScene built before; lib runs another
+    build of the oracle (load_from).
     Returns a dict of numpy outputs indexed like `pixels`."""
     from vkr import FRAME
-    lib = load()
+    lib = lib if lib is not None else load()
     sc = oscene if oscene is not None else OracleScene(scene, accel=accel)
     fr = np.ascontiguousarray(np.asarray(frame, FRAME).reshape(1))
     n = width * height if pixels is None else len(pixels)

@@ -59,7 +59,7 @@ def test_draw_matches_oracle(tmp_path):
 @pytest.mark.gpu
 def test_pose_replay_matches_oracle(tmp_path):
     """--poses replays recorded camera input: each frame's camera comes from
-    Camera::update of its pose (Source/Camera.cpp:91-107)."""
+    Camera::update of its pose (Source/Camera.cpp:48-64)."""
     import torch
     import vkr
     if not torch.cuda.is_available():

@@ -321,7 +321,7 @@ def test_jitter_frame(lib):
 
 
 # --------------------------------------------------------------------------
-# Camera::update (Source/Camera.cpp:91-107)
+# Camera::update (Source/Camera.cpp:48-64)
 # --------------------------------------------------------------------------
 def test_camera_update_closed_form(lib):
     cam = np.zeros(1, vkr.CAMERA)

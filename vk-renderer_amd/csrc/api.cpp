@@ -456,7 +456,7 @@ vkr_error vkr_debug_eval(vkr_ctx* ctx, uint32_t op, const float* in, float* out,
 
 vkr_error vkr_camera_from_angles(const float pos[3], float pitch, float yaw, vkr_camera* out)
 {
-    // Camera::update (Source/Camera.cpp:91-107) with GLM 0.9.9 float
+    // Camera::update (Source/Camera.cpp:48-64) with GLM 0.9.9 float
     // semantics: radians(x) = x * float(pi / 180), normalize(v) =
     // v * (1 / sqrt(dot(v, v))), dot = (x*x' + y*y') + z*z'.
     if (!pos || !out) return fail(VKR_INVALID_ARGUMENT, "NULL argument");

@@ -8,7 +8,7 @@
 namespace {
 inline vec3 madd(vec3 a, vec3 b, float s) { return vec3{a.x + b.x * s, a.y + b.y * s, a.z + b.z * s}; }
 
-// aux.front = glm::normalize(front) (Source/Camera.cpp:97-101), GLM float
+// aux.front = glm::normalize(front) (Source/Camera.cpp:52-58), GLM float
 // semantics: radians = deg * float(pi/180), normalize = v * (1/sqrt(dot)).
 vec3 front_of(float pitch, float yaw)
 {
@@ -34,7 +34,7 @@ void Camera::move_down(float speed) { data.pos = madd(data.pos, data.up, -speed)
 
 void Camera::update()
 {
-    // Source/Camera.cpp:93: the pitch is clamped in place.
+    // Source/Camera.cpp:50: the pitch is clamped in place.
     aux.pitch = aux.pitch < -89.0f ? -89.0f : (aux.pitch > 89.0f ? 89.0f : aux.pitch);
     const float pos[3] = {data.pos.x, data.pos.y, data.pos.z};
     vkr_camera c;

@@ -1,6 +1,6 @@
 // Camera.h - the reference's camera (Include/Camera.h:5-43), unchanged in
 // shape; the basis is computed by vkr_camera_from_angles (GLM float
-// semantics of Source/Camera.cpp:91-107).
+// semantics of Source/Camera.cpp:48-64).
 #pragma once
 
 struct vec3 {

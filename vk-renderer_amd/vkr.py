@@ -193,7 +193,7 @@ def check(code: int, where: str):
 
 
 def camera_from_angles(pos, pitch: float, yaw: float) -> np.ndarray:
-    """Camera::update (Source/Camera.cpp:91-107) through the library; returns a CAMERA record."""
+    """Camera::update (Source/Camera.cpp:48-64) through the library; returns a CAMERA record."""
     lib = load()
     p = (C.c_float * 3)(*[float(v) for v in pos])
     out = CameraC()
