@@ -1,19 +1,29 @@
 """Benchmark: Mrays/s (all bounces) of the gfx950 path kernel on one MI355X.
 
-Workload (one "step" = one frame): 1920x1080, 4 spp, max 4 bounces, 100k
-random triangles (vertices U[-50,50]^3, edges U[0.1,2]) + 256 random spheres
-+ 1 emissive light sphere + a ground plane, seed 0, FP32.  Rays counted are
-every trace_ray call of the reference program (camera, bounce and
-next-event shadow rays), read from the kernel's own counters.
+Headline workload (one "step" = one frame): 1920x1080, 4 spp, max 4
+bounces, 100k random triangles (vertices U[-50,50]^3, edges U[0.1,2]) + 256
+random spheres + 1 emissive light sphere + a ground plane, seed 0, FP32.
+Rays counted are every trace_ray call of the reference program (camera,
+bounce and next-event shadow rays), read from the kernel's own counters.
 
-Prints ONE JSON line.  Also reports:
-  roofline      algorithmic bytes per launch (BVH nodes, primitive records,
-                output) over the event-timed kernel duration vs 8 TB/s HBM;
-  cpu_baseline  the CPU restatement of the reference tracing the same BVH
-                as the GPU (oracle/bvh_accel.cpp, all host threads, results
-                identical to the linear scan) timed on a pixel sample of the
-                same frame; cpu_scan_baseline is the literal linear scan of
-                the reference over a smaller sample.
+Prints ONE JSON line.  Besides the headline it reports:
+  roofline      the dominant kernel (k_path) against HBM: measured HBM bytes
+                per launch (rocprof FETCH_SIZE + WRITE_SIZE, calibrated) over
+                the event-timed kernel; the algorithmic request bytes (BVH
+                nodes, primitive records, output), the measured L2 rate,
+                VALU issue utilisation and the node-loop SIMD efficiency
+                beside it, since the kernel is bound by issue and latency;
+  cpu_baseline  the CPU restatement of the reference tracing the same BVH as
+                the GPU (oracle/bvh_accel.cpp, results identical to the
+                linear scan) on the host threads this process may use (at
+                most 16, a GPU's share of its host) on a pixel sample of the
+                same frame; cpu_scan_baseline is the literal linear scan;
+  batch64       64 scenes (1e3..1e6 triangles, 256^2..1080p, 1..8 spp): one
+                plain launch per scene, one persistent launch per scene, and
+                ONE batched persistent launch replayed from a captured
+                hipGraph, at the default and at the autotuned shape;
+  heightfield   the 2.09M-triangle heightfield, 1080p, 8 spp, 8 bounces;
+  adjoint       the 256^2 / 16 spp adjoint workload, forward vs adjoint.
 """
 from __future__ import annotations
 
@@ -31,7 +41,13 @@ FP32_PEAK_TFLOPS = 78.6   # non-packed FP32 FMA: 256 CU x 128 FLOP/clk x 2.4 GHz
 # FP32 operations per unit of work (slab test of one box, one Moller-Trumbore
 # triangle test, one sphere test), for the secondary compute figure.
 BOX_FLOP, TRI_FLOP, SPHERE_FLOP = 20, 50, 20
+# Request bytes per unit of work as this build lays them out: a 64-byte node
+# (two child boxes), a 48-byte triangle record, a 16-byte sphere record plus
+# its 4-byte scan key, a plane's normal and offset, an rgba8 pixel.  The
+# std430 sizes of a Sphere and a Plane are 64 bytes each.
 NODE_B, TRI_B, SPHERE_B, PLANE_B, OUT_B = 64, 48, 20, 16, 4
+STD430_SPHERE_B = STD430_PLANE_B = 64
+CPU_THREAD_CAP = 16       # a GPU's share of its host's threads
 
 
 def parse():
@@ -47,6 +63,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batch64 section of the default run")
+    ap.add_argument("--no-heightfield", action="store_true", help="skip the heightfield section of the default run")
+    ap.add_argument("--batch-steps", type=int, default=2, help="timed repetitions per batch64 variant")
     ap.add_argument("--workload", default="soup100k", choices=["soup100k", "batch64", "bvhbuild"])
     ap.add_argument("--scenes", type=int, default=64, help="batch64: number of scenes")
     ap.add_argument("--device-bvh", action="store_true", help="trace scenes whose triangle BVH was built on the GPU")
@@ -59,6 +78,25 @@ def log(msg):
 
 
 METRIC = "Mrays/s (all bounces) at 1080p 100k-tri, plain vs batched vs adjoint, 1 GPU"
+
+
+def host_cpu():
+    """Host threads this process may use (capped at a GPU's share), the
+    machine's logical CPU count and its CPU model."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return min(CPU_THREAD_CAP, avail), os.cpu_count(), avail, model
 
 
 def bench_adjoint(ctx, torch, repeats=5):
@@ -95,29 +133,27 @@ def bench_adjoint(ctx, torch, repeats=5):
             "adjoint_mrays": round(rays / res["adjoint"] / 1e6, 2)}
 
 
-def bench_batch(a):
+def batch_section(ctx, torch, n_scenes=64, steps=2, block=0, chunk=0, device_bvh=False, tune=True):
     """64 scenes (1e3..1e6 triangles log-uniform, 256^2..1080p, 1..8 spp, 4
-    bounces) traced three ways: one plain launch per scene, one persistent
-    launch per scene, and ONE batched persistent launch (eager and replayed
-    from a captured hipGraph).  value = batched Mrays/s."""
-    import torch
-
+    bounces) traced: one plain launch per scene, one persistent launch per
+    scene, and ONE batched persistent launch (eager, and replayed from a
+    captured hipGraph), at the default shape and at the batch autotuner's.
+    Returns (result dict, seconds per batch of the tuned graph)."""
     import scenes
     import vkr
 
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
-    wls = scenes.batch_jobs(a.scenes)
-    ctx = vkr.Context()
+    wls = scenes.batch_jobs(n_scenes)
     hsids = [ctx.upload(wl.scene) for wl in wls]
     ctx.destroy_scene(ctx.upload(wls[0].scene, device_bvh=True))   # loads the build kernels
     dsids = [ctx.upload(wl.scene, device_bvh=True) for wl in wls]
     build_ms = {"host_sah": round(sum(ctx.info(s)["build_ms"] for s in hsids), 2),
                 "device": round(sum(ctx.info(s)["build_ms"] for s in dsids), 2)}
-    for s in (hsids if a.device_bvh else dsids):
+    for s in (hsids if device_bvh else dsids):
         ctx.destroy_scene(s)
-    sids = dsids if a.device_bvh else hsids
-    log(f"built {len(wls)} scenes in {time.perf_counter() - t0:.1f} s; BVH build ms (sum) {build_ms}; "
+    sids = dsids if device_bvh else hsids
+    log(f"batch: built {len(wls)} scenes in {time.perf_counter() - t0:.1f} s; BVH build ms (sum) {build_ms}; "
         f"tris {sum(len(w.scene.tris) for w in wls)}, pixels*spp {sum(w.width * w.height * w.spp for w in wls)}")
     outs = [torch.zeros((wl.width * wl.height, 4), dtype=torch.uint8, device=dev) for wl in wls]
 
@@ -126,42 +162,103 @@ def bench_batch(a):
             ctx.trace(sid, wl.frame, wl.params(sched=sched), rgba8=o)
 
     jobs = [ctx.job(sid, wl.frame, wl.params(), rgba8=o) for wl, sid, o in zip(wls, sids, outs)]
-    ctx.trace_batch(jobs, block=a.block, chunk=a.chunk)
+    ctx.trace_batch(jobs, block=block or 128, chunk=chunk or 8)
     torch.cuda.synchronize()
-    rays = ctx.counters()["rays"]
-    traced = ctx.counters()["rays_traced"]
-    gid = ctx.batch_graph(jobs, block=a.block, chunk=a.chunk)
+    cnt = ctx.counters()
+    rays, traced = cnt["rays"], cnt["rays_traced"]
+    g_default = ctx.batch_graph(jobs, block=block or 128, chunk=chunk or 8)
 
     def timed(fn):
         fn()
         torch.cuda.synchronize()
         ts = time.perf_counter()
-        for _ in range(a.steps):
+        for _ in range(steps):
             fn()
         torch.cuda.synchronize()
-        return (time.perf_counter() - ts) / a.steps
+        return (time.perf_counter() - ts) / steps
 
     res = {}
     res["plain_per_scene"] = timed(lambda: per_scene(vkr.SCHED_PLAIN))
+    log(f"batch: plain per scene {res['plain_per_scene']:.3f} s")
     res["persistent_per_scene"] = timed(lambda: per_scene(vkr.SCHED_PERSISTENT))
-    res["batched"] = timed(lambda: ctx.trace_batch(jobs, block=a.block, chunk=a.chunk))
-    res["batched_graph"] = timed(lambda: ctx.graph_launch(gid))
+    log(f"batch: persistent per scene {res['persistent_per_scene']:.3f} s")
+    res["batched"] = timed(lambda: ctx.trace_batch(jobs, block=block or 128, chunk=chunk or 8))
+    res["batched_graph"] = timed(lambda: ctx.graph_launch(g_default))
+    log(f"batch: batched {res['batched']:.3f} s, graph {res['batched_graph']:.3f} s")
+    ctx.graph_destroy(g_default)
+    tuned = None
+    if tune and not (block or chunk):
+        ts = time.perf_counter()
+        tuned = ctx.autotune_batch(jobs, repeats=1)
+        tune_s = time.perf_counter() - ts
+        g_tuned = ctx.batch_graph(jobs)          # block = chunk = 0: the tuned shape
+        res["batched_graph_tuned"] = timed(lambda: ctx.graph_launch(g_tuned))
+        ctx.graph_destroy(g_tuned)
+        log(f"batch: autotuned {tuned['block']}/{tuned['chunk']}/{tuned['stack']} in {tune_s:.1f} s, "
+            f"graph {res['batched_graph_tuned']:.3f} s")
+    best = res.get("batched_graph_tuned", res["batched_graph"])
     mr = {k: round(rays / v / 1e6, 2) for k, v in res.items()}
-    log(f"seconds per batch {res}\nMrays/s {mr}")
+    out = {
+        "workload": f"batch{len(wls)}_1e3-1e6tris_256sq-1080p_1-8spp_4bounce",
+        "rays_per_batch": rays, "rays_traversed_per_batch": traced,
+        "seconds_per_batch": {k: round(v, 4) for k, v in res.items()},
+        "mrays": mr,
+        "speedup_batched_vs_plain_per_scene": round(res["plain_per_scene"] / best, 3),
+        "speedup_batched_vs_persistent_per_scene": round(res["persistent_per_scene"] / best, 3),
+        "tuned_shape": None if tuned is None else {k: tuned[k] for k in ("block", "chunk", "stack")},
+        "tuned_vs_default_graph_ms": None if tuned is None else
+        [round(res["batched_graph_tuned"] * 1e3, 2), round(res["batched_graph"] * 1e3, 2)],
+        "autotune_table": None if tuned is None else tuned["table"],
+        "bvh_build_ms_all_scenes": build_ms, "bvh": "device" if device_bvh else "host_sah",
+    }
+    for s in sids:
+        ctx.destroy_scene(s)
+    return out, best
+
+
+def heightfield_section(ctx, torch, steps=3):
+    """2.09M-triangle fBm heightfield + 4096 metal/dielectric spheres + 1
+    light, 1920x1080, 8 spp, 8 bounces: event-timed frames."""
+    import scenes
+
+    wl = scenes.heightfield()
+    sid = ctx.upload(wl.scene)
+    info = ctx.info(sid)
+    rgba = torch.zeros((wl.width * wl.height, 4), dtype=torch.uint8, device="cuda")
+    p = wl.params()
+    ctx.trace(sid, wl.frame, p, rgba8=rgba)
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(steps):
+        ctx.trace(sid, wl.frame, p, rgba8=rgba)
+        ms.append(ctx.last_kernel_ms())
+    cnt = ctx.counters()
+    ctx.destroy_scene(sid)
+    k = sum(ms) / len(ms)
+    return {"workload": "heightfield2093058_1920x1080_8spp_8bounce", "n_tris": info["n_tris"],
+            "n_spheres": info["n_spheres"], "build_ms_host_sah": round(info["build_ms"], 2),
+            "kernel_ms": round(k, 3), "rays_per_frame": cnt["rays"], "rays_traversed_per_frame": cnt["rays_traced"],
+            "mrays": round(cnt["rays"] / k / 1e3, 2), "traversed_mrays": round(cnt["rays_traced"] / k / 1e3, 2)}
+
+
+def bench_batch(a):
+    """--workload batch64: the batch section on its own, as the JSON line."""
+    import torch
+
+    import vkr
+
+    ctx = vkr.Context()
+    res, best = batch_section(ctx, torch, a.scenes, max(1, a.steps), a.block, a.chunk, a.device_bvh)
     out = {
         "metric": METRIC,
-        "value": mr["batched_graph"], "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps, "warmup": 1,
-        "ms_per_step": round(res["batched_graph"] * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "value": res["mrays"]["batched_graph_tuned" if "batched_graph_tuned" in res["mrays"] else "batched_graph"],
+        "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps, "warmup": 1,
+        "ms_per_step": round(best * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded PCG64 scenes)",
-        "config": {"workload": f"batch{len(wls)}", "rays_per_batch": rays, "rays_traversed_per_batch": traced,
-                   "block": a.block, "chunk": a.chunk, "bvh": "device" if a.device_bvh else "host_sah"},
-        "bvh_build_ms_all_scenes": build_ms,
-        "variants_mrays": mr,
-        "speedup_batched_vs_plain_per_scene": round(res["plain_per_scene"] / res["batched_graph"], 3),
-        "speedup_batched_vs_persistent_per_scene": round(res["persistent_per_scene"] / res["batched_graph"], 3),
+        "config": {"workload": res["workload"], "block": a.block, "chunk": a.chunk},
+        "batch64": res,
     }
     print(json.dumps(out), flush=True)
-    ctx.graph_destroy(gid)
     ctx.close()
 
 
@@ -210,6 +307,15 @@ def bench_bvh_build(a):
            "config": {"workload": "bvhbuild_soup100k_heightfield2m"}, "results": res}
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def load_profile(name):
+    """A committed rocprof summary of the headline kernel (profiles/*.json)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
 
 
 def main():
@@ -268,7 +374,7 @@ def main():
     t0 = time.perf_counter()
     for i in range(a.steps):
         evs[i][0].record(stream)
-        ctx.trace(sid, wl.frame, p, rgba8=rgba)
+        ctx.trace(sid, wl.frame, p, rgba8=rgba)   # launched on torch's current stream
         evs[i][1].record(stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -279,47 +385,57 @@ def main():
     elapsed = t1 - t0
     rays = cnt["rays"]
     value = rays * a.steps / elapsed / 1e6
+    traversed_mrays = cnt["rays_traced"] * a.steps / elapsed / 1e6
     log(f"rays/frame {rays} shadow {cnt['shadow_rays']} overflow {cnt['window_overflow']} "
         f"launch ms mean {kern_ms:.3f} min {min(launch_ms):.3f} last kernel (lib events) {last_kernel_ms:.3f}")
 
-    algo_bytes = (work["nodes_visited"] * NODE_B + work["tris_tested"] * TRI_B +
-                  work["spheres_tested"] * SPHERE_B + work["planes_tested"] * PLANE_B + W * H * OUT_B)
-    achieved = algo_bytes / (kern_ms * 1e-3) / 1e9
+    # Request bytes of the traversal and shading (work counters of the same
+    # kernel's counting variant): mostly L2 hits, so not an HBM figure.
+    req_bytes = (work["nodes_visited"] * NODE_B + work["tris_tested"] * TRI_B +
+                 work["spheres_tested"] * SPHERE_B + work["planes_tested"] * PLANE_B + W * H * OUT_B)
+    req_bytes_std430 = (work["nodes_visited"] * NODE_B + work["tris_tested"] * TRI_B +
+                        work["spheres_tested"] * STD430_SPHERE_B + work["planes_tested"] * STD430_PLANE_B +
+                        W * H * OUT_B)
+    kern_s = kern_ms * 1e-3
     flops = (work["nodes_visited"] * 2 * BOX_FLOP + work["tris_tested"] * TRI_FLOP +
              work["spheres_tested"] * SPHERE_FLOP)
-    # The rocprof traffic files describe the 100k-triangle frame on the host
-    # builder's tree only.
-    measured = a.tris == 100_000 and not a.device_bvh
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_soup100k.json")
-    if measured and os.path.exists(tf):
-        try:
-            traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
-        except (ValueError, OSError):
-            traffic = None
-    l2_bytes = None
-    lf = os.path.join(ROOT, "profiles", "l2_soup100k.json")
-    if measured and os.path.exists(lf):
-        try:
-            d = json.load(open(lf))
-            l2_bytes = d["l2_read_bytes_per_launch"] + d["l2_write_bytes_per_launch"]
-        except (ValueError, OSError, KeyError):
-            l2_bytes = None
-    # The algorithmic bytes are mostly L2 hits (the node and primitive records
-    # of the 100k-triangle frame fit in L2/MALL), so `frac` near 1 does not
-    # mean HBM-bound; the measured HBM rate (rocprof traffic over the kernel
-    # time) is reported beside it.
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "hbm_gbs_measured": round(traffic / (kern_ms * 1e-3) / 1e9, 1) if traffic else None,
-                # Bytes the L2 serves per launch (rocprof sector counts) over this run's kernel time.
-                "l2_gbs_measured": round(l2_bytes / (kern_ms * 1e-3) / 1e9, 1) if l2_bytes else None,
-                "algorithmic_bytes_per_launch": int(algo_bytes),
-                "kernel_ms": round(kern_ms, 4),
-                # Secondary: traversal and intersection FP32 operations (two
-                # boxes per node visit) against the vector FP32 peak.
-                "fp32_tflops_est": round(flops / (kern_ms * 1e-3) / 1e12, 3),
-                "fp32_peak_tflops": FP32_PEAK_TFLOPS}
+    # The rocprof files describe the 100k-triangle frame on the host
+    # builder's tree with the default launch only.
+    measured = a.tris == 100_000 and not a.device_bvh and not (a.sched or a.block or a.chunk)
+    tp = load_profile("traffic_soup100k.json") if measured else None
+    lp = load_profile("l2_soup100k.json") if measured else None
+    traffic = tp.get("hbm_bytes_per_launch") if tp else None
+    l2_bytes = (lp["l2_read_bytes_per_launch"] + lp["l2_write_bytes_per_launch"]) if lp else None
+    hbm_gbs = traffic / kern_s / 1e9 if traffic else None
+    node_eff = work["nodes_visited"] / (64.0 * work["wave_node_iters"]) if work["wave_node_iters"] else None
+    roofline = {
+        "bound": "hbm",
+        # measured HBM bytes per launch over this run's event-timed kernel
+        "achieved": round(hbm_gbs, 1) if hbm_gbs else None,
+        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(hbm_gbs / HBM_PEAK_GBS, 4) if hbm_gbs else None,
+        "traffic": traffic,
+        "achieved_basis": "rocprof FETCH_SIZE + WRITE_SIZE per dispatch (calibrated on a copy) / kernel time",
+        "kernel_ms": round(kern_ms, 4),
+        # The kernel is not HBM-bound: it is bound by VALU issue and the
+        # latency of dependent node fetches (L2 hits).
+        "limiter": "VALU issue and dependent-fetch latency",
+        "valu_issue_util": tp.get("valu_issue_util") if tp else None,
+        "node_loop_simd_eff": round(node_eff, 4) if node_eff else None,
+        "leaf_simd_eff": round((work["tris_tested"] + work["spheres_tested"]) / (64.0 * work["wave_leaf_iters"]), 4)
+        if work["wave_leaf_iters"] else None,
+        "node_visits_per_traversed_ray": round(work["nodes_visited"] / max(1, work["rays_traced"]), 2),
+        "l2_gbs_measured": round(l2_bytes / kern_s / 1e9, 1) if l2_bytes else None,
+        "l2_hit_rate": tp.get("l2_hit_rate") if tp else None,
+        # Algorithmic request bytes per launch (node 64 B, triangle 48 B,
+        # sphere 20 B, plane 16 B, rgba8 4 B; with std430 spheres and planes
+        # of 64 B in request_bytes_std430_units): served mostly by L2.
+        "request_bytes_per_launch": int(req_bytes),
+        "request_gbs": round(req_bytes / kern_s / 1e9, 1),
+        "request_bytes_std430_units": int(req_bytes_std430),
+        "fp32_tflops_est": round(flops / kern_s / 1e12, 3),
+        "fp32_peak_tflops": FP32_PEAK_TFLOPS,
+    }
 
     variants = {}
     if not a.no_variants:
@@ -333,16 +449,27 @@ def main():
             torch.cuda.synchronize()
             variants[name] = round(ctx.counters()["rays"] * 3 / (time.perf_counter() - ts) / 1e6, 2)
         log(f"variants Mrays/s: {variants}")
+    ctx.destroy_scene(sid)
 
     adjoint = None
     if not a.no_adjoint:
         adjoint = bench_adjoint(ctx, torch)
         log(f"adjoint: {adjoint}")
 
+    heightfield = None
+    if not a.no_heightfield:
+        heightfield = heightfield_section(ctx, torch)
+        log(f"heightfield: {heightfield}")
+
+    batch = None
+    if not a.no_batch:
+        batch, _ = batch_section(ctx, torch, steps=max(1, a.batch_steps))
+        log(f"batch64: { {k: v for k, v in batch.items() if k != 'autotune_table'} }")
+
     cpu = cpu_scan = None
     if not a.no_cpu_baseline:
         import oracle_ffi
-        threads = min(16, os.cpu_count() or 1)
+        threads, nproc, avail, model = host_cpu()
 
         def cpu_leg(n_pix, accel, what):
             pix = np.random.default_rng(0).choice(W * H, n_pix, replace=False).astype(np.uint32)
@@ -352,9 +479,18 @@ def main():
                                  oscene=osc)
             dt = time.perf_counter() - ts
             n_rays = int(r["ray_count"].sum())
-            return {"value": round(n_rays / dt / 1e6, 6), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            v = n_rays / dt / 1e6
+            return {"value": round(v, 6), "unit": "Mrays/s", "cores": threads, "kind": "port",
                     "sample": f"{n_pix} random pixels of the same frame (all {wl.spp} spp x {wl.max_depth} "
-                              f"bounces, {n_rays} rays, {dt:.1f} s), {what}"}
+                              f"bounces, {n_rays} rays, {dt:.1f} s), {what}",
+                    "host_logical_cpus": nproc, "cpus_available": avail, "cpu_model": model,
+                    "threads_note": f"all {avail} CPUs available to this process, capped at {CPU_THREAD_CAP} "
+                                    f"(a GPU's share of its host)",
+                    "gpu_over_cpu": round(value / v, 1),
+                    # the CPU traces every shadow ray; the GPU skips those whose
+                    # contribution is exactly 0, so compare traversed rays too
+                    "gpu_traversed_mrays": round(traversed_mrays, 2),
+                    "gpu_traversed_over_cpu": round(traversed_mrays / v, 1)}
 
         cpu = cpu_leg(a.cpu_pixels, True, "the oracle tracing the same BVH as the GPU (scalar, one ray at a "
                                          "time, every shadow ray traced)")
@@ -381,7 +517,8 @@ def main():
                    "max_depth": wl.max_depth, "n_tris": info["n_tris"], "n_spheres": info["n_spheres"],
                    "n_planes": info["n_planes"], "n_emitters": info["n_emitters"],
                    "rays_per_frame": rays, "rays_traversed_per_frame": cnt["rays_traced"],
-                   "traversed_mrays": round(cnt["rays_traced"] * a.steps / elapsed / 1e6, 2),
+                   "traversed_mrays": round(traversed_mrays, 2),
+                   "window_overflow_per_frame": cnt["window_overflow"],
                    "sched": a.sched, "block": a.block, "chunk": a.chunk,
                    "bvh": "device" if a.device_bvh else "host_sah"},
         "bvh_build_ms": bvh_build_ms,
@@ -389,6 +526,8 @@ def main():
         "cpu_baseline": cpu,
         "cpu_scan_baseline": cpu_scan,
         "variants_mrays": variants,
+        "batch64": batch,
+        "heightfield": heightfield,
         "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)

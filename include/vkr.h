@@ -185,7 +185,8 @@ typedef struct vkr_scene_desc {
  * id owned by the context.  Planes stay a linear list. */
 vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* out_scene);
 
-/* Frees one scene. */
+/* Frees one scene.  Refused (VKR_INVALID_ARGUMENT) while a captured graph
+ * (vkr_batch_graph_create) traces it: destroy those graphs first. */
 vkr_error vkr_scene_destroy(vkr_ctx* ctx, uint32_t scene);
 
 typedef struct vkr_scene_info {
@@ -206,7 +207,10 @@ vkr_error vkr_scene_get_info(vkr_ctx* ctx, uint32_t scene, vkr_scene_info* out);
  *  roughness] followed by one global light scale L, so
  * n = 7 * n_spheres + 1.  The traced emissive of sphere s is L * emission_s.
  * Writes the materials of every sphere (albedo, emissive, roughness; type and
- * metalness stay) and recomputes the emitter list.  Host pointer.
+ * metalness stay) and recomputes the emitter list in place (its device
+ * buffer holds every sphere, so it never moves); captured graphs that trace
+ * the scene see the new list on their next launch.  Synchronises the device.
+ * Host pointer.
  * Build extension: the reference has no parameters (Tracer.comp:186-202). */
 vkr_error vkr_scene_set_sphere_params(vkr_ctx* ctx, uint32_t scene, const float* params, uint32_t n);
 
@@ -295,12 +299,17 @@ typedef struct vkr_scene_job {
 
 /* Build extension (no reference counterpart beyond Draw): traces every job in
  * ONE persistent launch whose waves pull (job, pixel-chunk) work items from a
- * global pool.  Results per job are identical to vkr_trace_frame. */
+ * global pool.  Results per job are identical to vkr_trace_frame.
+ * block = chunk = 0 takes vkr_autotune_batch's choice for this job set when
+ * there is one, else 128 lanes and a chunk of 8.  The pool entries (64 per
+ * 8x8 tile of every job) plus the resident lanes must stay below 2^32. */
 vkr_error vkr_trace_batch(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n_jobs,
                           uint32_t block, uint32_t chunk, void* stream);
 
 /* Captures a batched launch (pool reset + persistent kernel) in a hipGraph
- * owned by the context and returns its id; vkr_graph_launch replays it. */
+ * owned by the context and returns its id; vkr_graph_launch replays it.
+ * The graph holds copies of its scenes' device records: those scenes cannot
+ * be destroyed while it exists (vkr_scene_destroy refuses). */
 vkr_error vkr_batch_graph_create(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n_jobs,
                                  uint32_t block, uint32_t chunk, void* stream, uint32_t* out_graph);
 vkr_error vkr_graph_launch(vkr_ctx* ctx, uint32_t graph, void* stream);
@@ -329,6 +338,19 @@ typedef struct vkr_tune_result {
 vkr_error vkr_autotune(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* frame_data,
                        const vkr_trace_params* params, const vkr_outputs* outputs, uint32_t repeats,
                        vkr_tune_result* out);
+
+/* Build extension: the autotuner of batched launches (the reference's fixed
+ * shape is local_size 16x16 with vkCmdDispatch(RES/16, RES/16, 1),
+ * Assets/Tracer.comp:142 and Source/GraphicsDevice.cpp:1266).  Times
+ * vkr_trace_batch on these jobs (they write their outputs) by coordinate
+ * descent over workgroup size (64, 128, 256), pool chunk (1..32) and stack
+ * depth (32, 64 when the scenes allow 32), `repeats` timed launches per
+ * candidate, and remembers the fastest for later vkr_trace_batch and
+ * vkr_batch_graph_create calls on the same job set (scenes, image sizes,
+ * spp, depth and flags, in order) that pass block = chunk = 0.  Results do
+ * not depend on the shape.  Synchronous. */
+vkr_error vkr_autotune_batch(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n_jobs, uint32_t repeats,
+                             vkr_tune_result* out);
 
 /* Build extension: path-replay adjoint of the path kernel.  For loss gradient
  * d_radiance (device, W*H*4 floats, dLoss/d(mean radiance rgb); a ignored)

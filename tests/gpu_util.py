@@ -21,7 +21,11 @@ def gpu_trace(ctx, sid, wl, sched=0, block=0, chunk=0, flags=0, decisions=None, 
     p = wl.params(sched=sched, block=block, chunk=chunk, flags=flags, kernel=kernel)
     ctx.trace(sid, wl.frame, p, **out)
     torch.cuda.synchronize()
+    overflow = ctx.counters()["window_overflow"]
     res = {k: v.cpu().numpy() for k, v in out.items()}
+    # Closest-hit rays with more near-tied hits than the replay window holds
+    # (documented near-ties): a frame called bit-exact must have none.
+    res["window_overflow"] = overflow
     res["prim_id"] = res["prim_id"].view(np.uint32)
     res["ray_count"] = res["ray_count"].view(np.uint32)
     if "decisions" in res:
@@ -31,8 +35,10 @@ def gpu_trace(ctx, sid, wl, sched=0, block=0, chunk=0, flags=0, decisions=None, 
 
 def compare(g: dict, r: dict, pixels=None, exclude_x0_width=None) -> dict:
     """Agreement fractions between GPU outputs g (full frame) and oracle r."""
+    overflow = g.get("window_overflow")
     if pixels is not None:
-        g = {k: v[pixels] for k, v in g.items() if k != "decisions"}
+        overflow = None   # the count covers the whole frame, not the sample
+        g = {k: v[pixels] for k, v in g.items() if k not in ("decisions", "window_overflow")}
     rad_ok = (np.abs(g["radiance"] - r["radiance"]) <= 1e-4) | (np.isnan(g["radiance"]) & np.isnan(r["radiance"]))
     t_g, t_r = g["hit_t"], r["hit_t"]
     t_ok = (t_g.view(np.uint32) == t_r.view(np.uint32)) | (np.abs(t_g - t_r) <= 1e-5 * np.abs(t_r))
@@ -47,4 +53,6 @@ def compare(g: dict, r: dict, pixels=None, exclude_x0_width=None) -> dict:
         "ray_count": float((g["ray_count"] == r["ray_count"]).mean()),
         "max_rad_err": float(np.nanmax(np.abs(g["radiance"] - r["radiance"]))) if len(r["prim_id"]) else 0.0,
     }
+    if overflow is not None:
+        stats["window_overflow"] = int(overflow)
     return stats

@@ -204,3 +204,104 @@ def test_graph_set_frame_replays_new_push_constants(ctx, torch_cuda):
                 assert _same(torch, b[key], ref[key]), (k, key)
     finally:
         ctx.graph_destroy(gid)
+
+
+def test_autotune_batch_is_bit_identical_to_default(ctx, torch_cuda):
+    """The batch autotuner (workgroup size, pool chunk, stack depth, chosen
+    for this job set) only changes the launch shape: the tuned batch, eager
+    and captured, equals the default-shape batch bit for bit."""
+    torch = torch_cuda
+    wls = _small_jobs()
+    sids = [ctx.upload(wl.scene) for wl in wls]
+    default = [_bufs(torch, wl) for wl in wls]
+    ctx.trace_batch([ctx.job(sid, wl.frame, wl.params(), **b) for wl, sid, b in zip(wls, sids, default)],
+                    block=128, chunk=8)
+    tuned = [_bufs(torch, wl) for wl in wls]
+    jobs = [ctx.job(sid, wl.frame, wl.params(), **b) for wl, sid, b in zip(wls, sids, tuned)]
+    res = ctx.autotune_batch(jobs, repeats=1)
+    print(f"\n[autotune batch] block {res['block']} chunk {res['chunk']} stack {res['stack']} "
+          f"best {res['best_ms']:.3f} ms over {len(res['table'])} candidates", flush=True)
+    assert res["block"] in (64, 128, 256) and res["chunk"] in (1, 4, 8, 16, 32) and res["stack"] in (32, 64)
+    assert len(res["table"]) >= 7
+    assert {t["block"] for t in res["table"]} == {64, 128, 256}
+    assert abs(res["best_ms"] - min(t["ms"] for t in res["table"])) < 1e-3
+    for b in tuned:
+        for v in b.values():
+            v.zero_()
+    ctx.trace_batch(jobs)                      # block = chunk = 0: the tuned shape
+    torch.cuda.synchronize()
+    for d, t in zip(default, tuned):
+        for k in KEYS:
+            assert _same(torch, d[k], t[k]), k
+    for b in tuned:
+        for v in b.values():
+            v.zero_()
+    gid = ctx.batch_graph(jobs)
+    ctx.graph_launch(gid)
+    torch.cuda.synchronize()
+    for d, t in zip(default, tuned):
+        for k in KEYS:
+            assert _same(torch, d[k], t[k]), k
+    ctx.graph_destroy(gid)
+    for sid in sids:
+        ctx.destroy_scene(sid)
+
+
+def test_graph_follows_sphere_parameter_changes(ctx, torch_cuda):
+    """A captured graph keeps tracing correctly when vkr_scene_set_sphere_params
+    changes the scene's emitter list (1 -> 3 -> 6 -> 0 emitters): each replay
+    equals vkr_trace_frame with the same parameters.  The scene cannot be
+    destroyed while the graph exists."""
+    torch = torch_cuda
+    rng = np.random.default_rng(21)
+    sph = scenes.random_spheres(rng, 10, -6.0, 6.0, kinds=("diffuse", "metal"))
+    sc = vkr.Scene(spheres=sph, planes=scenes.ground(-7.0))
+    theta = scenes.sphere_params(sph, light_scale=1.0)
+    n_sph = len(sph)
+
+    def with_emitters(k):
+        t = theta.copy()
+        for s in range(n_sph):
+            t[7 * s + 3:7 * s + 6] = (40.0, 30.0, 20.0) if s < k else (0.0, 0.0, 0.0)
+        return t
+
+    cam = vkr.camera_from_angles((0.0, 2.0, -22.0), -5.0, 90.0)
+    wl = scenes.Workload("emitter_changes", sc, scenes.frame(cam, np.float32(64) / np.float32(48)), 64, 48, 2, 4)
+    sid = ctx.upload(sc)
+    ctx.set_sphere_params(sid, with_emitters(1))
+    b = _bufs(torch, wl)
+    gid = ctx.batch_graph([ctx.job(sid, wl.frame, wl.params(), **b)])
+    try:
+        with pytest.raises(vkr.VkrError) as err:
+            ctx.destroy_scene(sid)
+        assert err.value.code == 4
+        for k in (1, 3, 6, 0):
+            ctx.set_sphere_params(sid, with_emitters(k))
+            assert ctx.info(sid)["n_emitters"] == k
+            for v in b.values():
+                v.zero_()
+            ctx.graph_launch(gid)
+            ref = _bufs(torch, wl)
+            ctx.trace(sid, wl.frame, wl.params(), **ref)
+            torch.cuda.synchronize()
+            for key in KEYS:
+                assert _same(torch, b[key], ref[key]), (k, key)
+    finally:
+        ctx.graph_destroy(gid)
+    ctx.destroy_scene(sid)
+
+
+def test_batch_pool_cursor_limit(ctx, torch_cuda):
+    """The 32-bit pool cursor can run past the pool's end by the resident
+    lanes: a batch whose entries come within that margin of 2^32 is refused
+    before anything is launched (eager and captured)."""
+    wl = scenes.reference_scene(8, 8, spp=1, max_depth=1)
+    sid = ctx.upload(wl.scene)
+    big = vkr.Context.params(16384, 16384, 1, 1)          # 2^28 pool entries
+    last = vkr.Context.params(16384, 16376, 1, 1)         # 2^28 - 2^17
+    jobs = [ctx.job(sid, wl.frame, big) for _ in range(15)] + [ctx.job(sid, wl.frame, last)]
+    for call in (lambda: ctx.trace_batch(jobs, block=128), lambda: ctx.batch_graph(jobs, block=128)):
+        with pytest.raises(vkr.VkrError) as err:
+            call()
+        assert err.value.code == 4 and "2^32" in str(err.value)
+    ctx.destroy_scene(sid)

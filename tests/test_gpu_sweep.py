@@ -90,6 +90,7 @@ def test_random_scene_matches_oracle(ctx, torch_cuda, seed, device_bvh, variant)
     r = oracle_ffi.trace(wl.scene, wl.frame, wl.width, wl.height, wl.spp, wl.max_depth, epsilon=eps)
     st = compare(g, r)
     print(f"[sweep {seed}] " + " ".join(f"{k}={v}" for k, v in st.items()), flush=True)
+    assert st["window_overflow"] == 0, st
     assert st["prim_id"] == 1.0 and st["hit_t_bits"] == 1.0 and st["ray_count"] == 1.0, st
     assert st["radiance_1e-4"] == 1.0 and st["rgba8"] == 1.0, st
     ctx.destroy_scene(sid)

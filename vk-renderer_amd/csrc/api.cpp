@@ -226,7 +226,10 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
     VKR_HIP(sc->tri_mat.upload(desc->tri_material ? desc->tri_material : &kMirror, sizeof(vkr_material)));
     const uint32_t nm = (desc->flags & VKR_SCENE_TRI_MATERIAL_INDEX) ? desc->n_materials : 0u;
     VKR_HIP(sc->materials.upload(nm ? desc->materials : nullptr, (size_t)nm * sizeof(vkr_material)));
-    VKR_HIP(sc->emitters.upload(em.data(), em.size() * sizeof(uint32_t)));
+    // Room for every sphere: vkr_scene_set_sphere_params rewrites the list in
+    // place, so its device address never changes while graphs hold it.
+    VKR_HIP(sc->emitters.alloc(std::max<size_t>(desc->n_spheres, 1u) * sizeof(uint32_t)));
+    if (!em.empty()) VKR_HIP(hipMemcpy(sc->emitters.p, em.data(), em.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
 
     DevScene& d = sc->dev;
     d.nodes = sc->nodes.as<BvhNode>();
@@ -282,6 +285,10 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
 vkr_error vkr_scene_destroy(vkr_ctx* ctx, uint32_t scene)
 {
     if (!get_scene(ctx, scene)) return fail(VKR_INVALID_ARGUMENT, "unknown scene");
+    // A captured graph traces from copies of the scene's device pointers.
+    for (const auto& g : ctx->graphs)
+        if (g && std::find(g->job_scene.begin(), g->job_scene.end(), scene) != g->job_scene.end())
+            return fail(VKR_INVALID_ARGUMENT, "a captured graph traces this scene; destroy the graph first");
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
     ctx->scenes[scene].reset();
@@ -316,17 +323,24 @@ vkr_error vkr_scene_set_sphere_params(vkr_ctx* ctx, uint32_t scene, const float*
     const std::vector<uint32_t> em = emitter_list(s->host_spheres);
     VKR_HIP(hipDeviceSynchronize());
     VKR_HIP(hipMemcpy(s->spheres.p, s->host_spheres.data(), (size_t)ns * sizeof(vkr_sphere), hipMemcpyHostToDevice));
-    if (em.size() * sizeof(uint32_t) > s->emitters.bytes) {
-        VKR_HIP(s->emitters.upload(em.data(), em.size() * sizeof(uint32_t)));
-        s->dev.emitters = s->emitters.as<uint32_t>();
-    } else if (!em.empty()) {
+    // The list was sized for every sphere at upload: it is rewritten in place.
+    if (!em.empty())
         VKR_HIP(hipMemcpy(s->emitters.p, em.data(), em.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    }
     s->dev.n_emitters = (uint32_t)em.size();
     s->dev.emitters_bounded = emitters_bounded(s->host_spheres, em);
     s->info.n_emitters = (uint32_t)em.size();
     s->host_theta.assign(params, params + n);
     VKR_HIP(hipMemcpy(s->theta.p, s->host_theta.data(), (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+    // Captured graphs hold a copy of the scene record per job: refresh the
+    // emitter count and bounds there too (the device is idle here).
+    for (const auto& g : ctx->graphs) {
+        if (!g) continue;
+        for (uint32_t j = 0; j < g->n_jobs; ++j) {
+            if (g->job_scene[j] != scene) continue;
+            char* dst = static_cast<char*>(g->jobs.p) + (size_t)j * sizeof(JobDesc) + offsetof(JobDesc, sc);
+            VKR_HIP(hipMemcpy(dst, &s->dev, sizeof(DevScene), hipMemcpyHostToDevice));
+        }
+    }
     return VKR_SUCCESS;
 }
 

@@ -84,6 +84,66 @@ uint32_t batch_grid(const BatchPlan& plan, const PathVariant& v, int device)
 
 bool valid_block(uint32_t b) { return b == 0 || b == 64 || b == 128 || b == 256; }
 
+// The 32-bit pool cursor: once the pool is empty, every resident wave that
+// has not yet seen it run dry still claims up to its 64 lanes' worth, so the
+// cursor can pass the total by at most the number of resident lanes.
+vkr_error check_cursor(const BatchPlan& plan, uint32_t grid, uint32_t block)
+{
+    if (plan.total + (uint64_t)grid * block > 0xFFFFFFFFull)
+        return fail(VKR_INVALID_ARGUMENT, "batch too large: pool entries plus resident lanes must stay below 2^32");
+    return VKR_SUCCESS;
+}
+
+// A batch's job set, as the batch autotuner keys its result: every job's
+// scene, image size, spp, depth and flags, in order (FNV-1a).
+uint64_t batch_key(const vkr_scene_job* jobs, uint32_t n)
+{
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](uint32_t v) {
+        for (int k = 0; k < 4; ++k) {
+            h ^= (v >> (8 * k)) & 0xFFu;
+            h *= 1099511628211ull;
+        }
+    };
+    mix(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const vkr_scene_job& j = jobs[i];
+        mix(j.scene);
+        mix(j.params.width);
+        mix(j.params.height);
+        mix(j.params.spp);
+        mix(j.params.max_depth);
+        mix(j.params.flags);
+    }
+    return h;
+}
+
+struct BatchShape {
+    uint32_t block, chunk;
+};
+
+// The launch shape of a batch: the caller's block and chunk, else the batch
+// autotuner's choice for this job set (which may also raise the stack
+// depth), else 128 lanes and the default chunk.
+BatchShape batch_shape(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n, BatchPlan& plan, uint32_t block,
+                       uint32_t chunk)
+{
+    BatchShape s{block, chunk};
+    if (block == 0 && chunk == 0) {
+        const uint64_t key = batch_key(jobs, n);
+        for (const BatchTuneEntry& t : ctx->tuned_batch) {
+            if (t.key != key) continue;
+            s.block = t.block;
+            s.chunk = t.chunk;
+            plan.stack = std::max(plan.stack, t.stack);
+            break;
+        }
+    }
+    if (!s.block) s.block = 128u;
+    if (!s.chunk) s.chunk = kDefaultChunk;
+    return s;
+}
+
 }  // namespace
 
 extern "C" {
@@ -124,14 +184,16 @@ vkr_error vkr_trace_batch(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n_jo
 
     DevCounters* cnt = ctx->counters.as<DevCounters>();
     uint32_t* pool = ctx->pool.as<uint32_t>();
-    const uint32_t blk = block ? block : 128u;
     const bool count = (ctx->flags & VKR_CTX_COUNT_WORK) != 0;
-    PathLaunch a = batch_launch(plan, ctx->jobs.as<JobDesc>(), ctx->job_start.as<uint32_t>(), n_jobs, chunk, pool,
+    const BatchShape sh = batch_shape(ctx, jobs, n_jobs, plan, block, chunk);
+    PathLaunch a = batch_launch(plan, ctx->jobs.as<JobDesc>(), ctx->job_start.as<uint32_t>(), n_jobs, sh.chunk, pool,
                                 cnt);
-    const PathVariant v = batch_variant(plan, blk, count);
+    const PathVariant v = batch_variant(plan, sh.block, count);
     const uint32_t grid = batch_grid(plan, v, ctx->device);
+    const vkr_error ce = check_cursor(plan, grid, sh.block);
+    if (ce != VKR_SUCCESS) return ce;
     if (v.pool) {
-        const vkr_error le = ensure_lanes(ctx, (size_t)grid * blk);
+        const vkr_error le = ensure_lanes(ctx, (size_t)grid * sh.block);
         if (le != VKR_SUCCESS) return le;
         a.lanes = ctx->lanes.as<char>();
     }
@@ -153,6 +215,13 @@ vkr_error vkr_batch_graph_create(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32
     const vkr_error pe = plan_batch(ctx, jobs, n_jobs, plan);
     if (pe != VKR_SUCCESS) return pe;
     VKR_HIP(hipSetDevice(ctx->device));
+    const bool count = (ctx->flags & VKR_CTX_COUNT_WORK) != 0;
+    const BatchShape sh = batch_shape(ctx, jobs, n_jobs, plan, block, chunk);
+    const PathVariant v = batch_variant(plan, sh.block, count);
+    const uint32_t grid = batch_grid(plan, v, ctx->device);
+    const vkr_error ce = check_cursor(plan, grid, sh.block);
+    if (ce != VKR_SUCCESS) return ce;
+
     auto g = std::make_unique<Graph>();
     VKR_HIP(g->jobs.upload(plan.jobs.data(), plan.jobs.size() * sizeof(JobDesc)));
     VKR_HIP(g->job_start.upload(plan.start.data(), plan.start.size() * sizeof(uint32_t)));
@@ -160,21 +229,19 @@ vkr_error vkr_batch_graph_create(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32
     VKR_HIP(g->counters.alloc(sizeof(DevCounters)));
     g->total = plan.total;
     g->n_jobs = n_jobs;
+    g->job_scene.resize(n_jobs);
+    for (uint32_t i = 0; i < n_jobs; ++i) g->job_scene[i] = jobs[i].scene;
+    PathLaunch a = batch_launch(plan, g->jobs.as<JobDesc>(), g->job_start.as<uint32_t>(), n_jobs, sh.chunk,
+                                g->pool.as<uint32_t>(), g->counters.as<DevCounters>());
+    if (v.pool) {
+        VKR_HIP(g->lanes.alloc((size_t)grid * sh.block * kLaneBytes));   // the graph's own records
+        a.lanes = g->lanes.as<char>();
+    }
 
     // Capture on a private stream: a fill kernel resets the pool cursor and the
     // counters (no memset nodes), then the persistent batch kernel.
     hipStream_t cap = nullptr;
     VKR_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-    const uint32_t blk = block ? block : 128u;
-    const bool count = (ctx->flags & VKR_CTX_COUNT_WORK) != 0;
-    PathLaunch a = batch_launch(plan, g->jobs.as<JobDesc>(), g->job_start.as<uint32_t>(), n_jobs, chunk,
-                                g->pool.as<uint32_t>(), g->counters.as<DevCounters>());
-    const PathVariant v = batch_variant(plan, blk, count);
-    const uint32_t grid = batch_grid(plan, v, ctx->device);
-    if (v.pool) {
-        VKR_HIP(g->lanes.alloc((size_t)grid * blk * kLaneBytes));   // the graph's own records
-        a.lanes = g->lanes.as<char>();
-    }
     hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
         const hipError_t e1 = launch_zero_counters(g->counters.as<DevCounters>(), g->pool.as<uint32_t>(), cap);
@@ -305,6 +372,76 @@ vkr_error vkr_autotune(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd, c
     out->chunk = win.chunk;
     out->stack = win.stack;
     out->sched = win.sched;
+    out->best_ms = best;
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_autotune_batch(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n_jobs, uint32_t repeats,
+                             vkr_tune_result* out)
+{
+    if (!ctx || !jobs || !out) return fail(VKR_INVALID_ARGUMENT, "NULL argument");
+    BatchPlan plan;
+    const vkr_error pe = plan_batch(ctx, jobs, n_jobs, plan);
+    if (pe != VKR_SUCCESS) return pe;
+    VKR_HIP(hipSetDevice(ctx->device));
+    if (repeats == 0) repeats = 1;
+    std::memset(out, 0, sizeof(*out));
+    const uint64_t key = batch_key(jobs, n_jobs);
+    ctx->tuned_batch.erase(std::remove_if(ctx->tuned_batch.begin(), ctx->tuned_batch.end(),
+                                          [&](const BatchTuneEntry& t) { return t.key == key; }),
+                           ctx->tuned_batch.end());
+    const uint32_t needed = plan.stack;
+
+    // One candidate: installed as this job set's tuning so that the launch
+    // takes its stack depth too, then timed over `repeats` launches.
+    auto time_shape = [&](uint32_t b, uint32_t c, uint32_t st, float& ms) -> vkr_error {
+        ctx->tuned_batch.push_back(BatchTuneEntry{key, b, c, st});
+        float total = 0.0f;
+        vkr_error e = VKR_SUCCESS;
+        for (uint32_t r = 0; r < repeats && e == VKR_SUCCESS; ++r) {
+            e = vkr_trace_batch(ctx, jobs, n_jobs, 0, 0, nullptr);
+            float t = 0.0f;
+            if (e == VKR_SUCCESS) e = vkr_last_kernel_ms(ctx, &t);
+            total += t;
+        }
+        ctx->tuned_batch.pop_back();
+        ms = total / (float)repeats;
+        if (e == VKR_SUCCESS && out->n_tried < 64) {
+            const uint32_t k = out->n_tried++;
+            out->cand_block[k] = b;
+            out->cand_chunk[k] = c;
+            out->cand_stack[k] = st;
+            out->cand_sched[k] = VKR_SCHED_PERSISTENT;
+            out->cand_ms[k] = ms;
+        }
+        return e;
+    };
+
+    // Coordinate descent (one batch launch is long, a full grid is not
+    // affordable): workgroup size at the default chunk, then the pool chunk,
+    // then the stack depth.  One warm launch loads the kernels first.
+    uint32_t best_b = 128u, best_c = kDefaultChunk, best_s = needed;
+    float best = 3.4e38f, ms = 0.0f;
+    vkr_error e = vkr_trace_batch(ctx, jobs, n_jobs, best_b, best_c, nullptr);
+    if (e != VKR_SUCCESS) return e;
+    for (uint32_t b : {64u, 128u, 256u}) {
+        if ((e = time_shape(b, best_c, best_s, ms)) != VKR_SUCCESS) return e;
+        if (ms < best) { best = ms; best_b = b; }
+    }
+    for (uint32_t c : {1u, 4u, 16u, 32u}) {
+        if ((e = time_shape(best_b, c, best_s, ms)) != VKR_SUCCESS) return e;
+        if (ms < best) { best = ms; best_c = c; }
+    }
+    for (uint32_t st : {32u, 64u}) {
+        if (st <= needed) continue;
+        if ((e = time_shape(best_b, best_c, st, ms)) != VKR_SUCCESS) return e;
+        if (ms < best) { best = ms; best_s = st; }
+    }
+    ctx->tuned_batch.push_back(BatchTuneEntry{key, best_b, best_c, best_s});
+    out->block = best_b;
+    out->chunk = best_c;
+    out->stack = best_s;
+    out->sched = VKR_SCHED_PERSISTENT;
     out->best_ms = best;
     return VKR_SUCCESS;
 }

@@ -86,6 +86,7 @@ struct Graph {
     DevBuf jobs, job_start, pool, counters, lanes;
     uint64_t total = 0;
     uint32_t n_jobs = 0;
+    std::vector<uint32_t> job_scene;   // scene id of every job: the graph's table holds copies of their DevScene
     ~Graph()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -98,6 +99,12 @@ struct Graph {
 struct TuneEntry {
     uint32_t scene, width, height;
     uint32_t block, chunk, stack, sched;
+};
+
+// Autotuned launch shape for one batch job set (batch.cpp: batch_key).
+struct BatchTuneEntry {
+    uint64_t key;
+    uint32_t block, chunk, stack;
 };
 
 }  // namespace vkr
@@ -117,6 +124,7 @@ struct vkr_ctx {
     bool timed = false;
     const vkr::DevCounters* last_counters = nullptr;   // NULL = ctx counters
     std::vector<vkr::TuneEntry> tuned;
+    std::vector<vkr::BatchTuneEntry> tuned_batch;
 };
 
 namespace vkr {

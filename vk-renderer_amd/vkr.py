@@ -168,6 +168,7 @@ def load(path: str = LIB_PATH):
         "vkr_camera_from_angles": (e, [C.POINTER(C.c_float), C.c_float, C.c_float, C.POINTER(CameraC)]),
         "vkr_debug_eval": (e, [vp, u32, vp, vp, u32, vp]),
         "vkr_autotune": (e, [vp, u32, vp, C.POINTER(TraceParams), C.POINTER(Outputs), u32, C.POINTER(TuneResult)]),
+        "vkr_autotune_batch": (e, [vp, C.POINTER(SceneJob), u32, u32, C.POINTER(TuneResult)]),
         "vkr_scene_save": (e, [C.c_char_p, C.POINTER(SceneDesc)]),
         "vkr_scene_load": (e, [C.c_char_p, C.POINTER(SceneDesc)]),
         "vkr_scene_free": (None, [C.POINTER(SceneDesc)]),
@@ -396,6 +397,17 @@ class Context:
                       ms=round(float(r.cand_ms[i]), 4)) for i in range(r.n_tried)]
         return dict(block=r.block, chunk=r.chunk, stack=r.stack, sched=r.sched, best_ms=float(r.best_ms),
                     table=table)
+
+    def autotune_batch(self, jobs, repeats: int = 1):
+        """Runs the batch autotuner on a job set (the jobs write their
+        outputs); later trace_batch / batch_graph calls on the same job set
+        with block = chunk = 0 use the winner."""
+        arr = (SceneJob * len(jobs))(*jobs)
+        r = TuneResult()
+        check(self.lib.vkr_autotune_batch(self.h, arr, len(jobs), repeats, C.byref(r)), "vkr_autotune_batch")
+        table = [dict(block=r.cand_block[i], chunk=r.cand_chunk[i], stack=r.cand_stack[i],
+                      ms=round(float(r.cand_ms[i]), 4)) for i in range(r.n_tried)]
+        return dict(block=r.block, chunk=r.chunk, stack=r.stack, best_ms=float(r.best_ms), table=table)
 
     def adjoint(self, sid: int, frame: np.ndarray, params: TraceParams, d_radiance, d_params, stream=None):
         fr = np.ascontiguousarray(np.asarray(frame, FRAME).reshape(1))
