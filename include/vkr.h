@@ -362,6 +362,27 @@ vkr_error vkr_adjoint(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* frame_
                       const vkr_trace_params* params, const float* d_radiance,
                       float* d_params, void* stream);
 
+/* The path kernel without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * results and counters as vkr_trace_frame, but the kernel's scratch (work
+ * counters, the pool cursor and the per-lane ray records of the wave-shared
+ * ray pool) lives in a static device array of VKR_STATIC_SCRATCH_BYTES that
+ * every GPU the library runs on holds until the process ends; this call
+ * allocates nothing and records no timing events.  The persistent grid is
+ * capped to the lanes whose records fit (VKR_STATIC_SCRATCH_LANE_BYTES);
+ * params->sched == VKR_SCHED_PLAIN with the ray pool (one record per pixel)
+ * is refused with VKR_INVALID_ARGUMENT when the records would not fit.
+ * Calls share their GPU's array: calls that overlap on one GPU (on
+ * different streams, or from several contexts or threads) race, and
+ * vkr_last_counters reads the last call's counters.  Asynchronous on
+ * `stream`; the scenes and outputs are the caller's, as for
+ * vkr_trace_frame. */
+#define VKR_STATIC_SCRATCH_LANE_BYTES (64u << 20)
+#define VKR_STATIC_SCRATCH_BYTES (512u + VKR_STATIC_SCRATCH_LANE_BYTES)
+vkr_error vkr_trace_frame_static(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* frame_data,
+                                 const vkr_trace_params* params, const vkr_outputs* outputs, void* stream);
+
 /* Counters of the last trace/batch/adjoint call on this context (filled when
  * the call's stream completes; vkr_last_counters synchronises). */
 typedef struct vkr_counters {

@@ -112,6 +112,9 @@ vkr_error vkr_create(const vkr_create_info* info, vkr_ctx** out_ctx)
     VKR_HIP(ctx->counters.alloc(sizeof(DevCounters)));
     VKR_HIP(ctx->pool.alloc(64));
     VKR_HIP(hipMemset(ctx->counters.p, 0, sizeof(DevCounters)));
+    // The device address of this device's static scratch (vkr_trace_frame_static).
+    char* scratch = nullptr;
+    VKR_HIP(static_scratch(dev, &scratch));
     *out_ctx = ctx.release();
     return VKR_SUCCESS;
 }
@@ -411,7 +414,87 @@ vkr_error vkr_trace_frame(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd
         VKR_HIP(hipEventRecord(ctx->ev1, st));
     }
     ctx->timed = true;
+    ctx->last_static = false;
     ctx->last_counters = nullptr;
+    return VKR_SUCCESS;
+}
+
+vkr_error vkr_trace_frame_static(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd,
+                                 const vkr_trace_params* params, const vkr_outputs* outputs, void* stream)
+{
+    // Allocates nothing: no hipMalloc, no events, no growth of context
+    // buffers.  The kernels' scratch is the device's static array.
+    if (!ctx || !fd || !outputs) return fail(VKR_INVALID_ARGUMENT, "NULL argument");
+    const vkr_error pe = validate_params(params);
+    if (pe != VKR_SUCCESS) return pe;
+    if ((params->flags & (VKR_TRACE_RECORD_DECISIONS | VKR_TRACE_REPLAY_DECISIONS)) && !outputs->decisions)
+        return fail(VKR_INVALID_ARGUMENT, "decision flags need outputs.decisions");
+    if ((params->flags & (VKR_TRACE_RECORD_DECISIONS | VKR_TRACE_REPLAY_DECISIONS)) && params->max_depth > 8)
+        return fail(VKR_INVALID_ARGUMENT, "decision record/replay supports max_depth <= 8");
+    Scene* s = nullptr;
+    if (params->kernel == VKR_KERNEL_PATH) {
+        s = get_scene(ctx, scene);
+        if (!s) return fail(VKR_INVALID_ARGUMENT, "unknown scene");
+    }
+    VKR_HIP(hipSetDevice(ctx->device));
+    char* scratch = nullptr;
+    VKR_HIP(static_scratch(ctx->device, &scratch));
+    DevCounters* cnt = reinterpret_cast<DevCounters*>(scratch);
+    uint32_t* pool = reinterpret_cast<uint32_t*>(scratch + kStaticPoolOff);
+    char* lanes = scratch + kStaticLanesOff;
+    const size_t max_lanes = (size_t)VKR_STATIC_SCRATCH_LANE_BYTES / kLaneBytes;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    static const Scene kNoScene{};
+    JobDesc j = make_job(s ? *s : kNoScene, *fd, *params, outputs);
+    if (params->kernel == VKR_KERNEL_WHITTED) {
+        VKR_HIP(launch_zero_counters(cnt, pool, st));
+        VKR_HIP(launch_whitted(j, cnt, st));
+    } else {
+        PathLaunch a{};
+        a.job = j;
+        a.total = j.n_tiles * 64u;
+        a.pool = pool;
+        a.counters = cnt;
+        uint32_t block = params->block, chunk = params->chunk, sched = params->sched;
+        uint32_t stack = s->dev.stack_needed;
+        if (const TuneEntry* t = find_tuned(ctx, scene, *params)) {
+            if (!block) block = t->block;
+            if (!chunk) chunk = t->chunk;
+            if (sched == VKR_SCHED_AUTO) sched = t->sched;
+            stack = std::max(stack, t->stack);
+        }
+        if (!block) block = 128u;
+        const bool persistent = sched != VKR_SCHED_PLAIN;
+        a.persistent = persistent ? 1u : 0u;
+        a.chunk = chunk ? chunk : kDefaultChunk;
+        const PathVariant v{block, stack, (ctx->flags & VKR_CTX_COUNT_WORK) != 0, false,
+                            use_ray_pool(params->flags, s->dev.n_emitters)};
+        const uint32_t step = persistent ? a.total : kMaxPlainLanes;
+        uint32_t grid = persistent ? std::min(persistent_grid(v, ctx->device), (a.total + block - 1u) / block)
+                                   : (std::min(a.total, step) + block - 1u) / block;
+        if (v.pool) {
+            // The ray records of every resident lane must fit the array: a
+            // persistent grid is capped to them, a one-pixel-per-lane launch
+            // (one record per pixel) is refused when they do not.
+            const uint32_t cap = (uint32_t)(max_lanes / block);
+            if (!persistent && (size_t)grid * block > max_lanes)
+                return fail(VKR_INVALID_ARGUMENT,
+                            "VKR_SCHED_PLAIN with the ray pool needs one ray record per pixel, more than the static "
+                            "scratch holds (VKR_STATIC_SCRATCH_LANE_BYTES); use the persistent schedule");
+            if (cap == 0) return fail(VKR_INVALID_ARGUMENT, "workgroup too large for the static scratch");
+            grid = std::min(grid, cap);
+            a.lanes = lanes;
+        }
+        VKR_HIP(launch_zero_counters(cnt, pool, st));
+        for (uint32_t base = 0; base < a.total; base += step) {
+            a.base = base;
+            const uint32_t g = persistent ? grid : (std::min(a.total - base, step) + block - 1u) / block;
+            VKR_HIP(launch_path(a, v, g, st));
+        }
+    }
+    ctx->timed = false;
+    ctx->last_static = true;
+    ctx->last_counters = cnt;
     return VKR_SUCCESS;
 }
 
@@ -420,6 +503,7 @@ vkr_error vkr_last_counters(vkr_ctx* ctx, vkr_counters* out)
     if (!ctx || !out) return fail(VKR_INVALID_ARGUMENT, "NULL argument");
     VKR_HIP(hipSetDevice(ctx->device));
     if (ctx->timed) VKR_HIP(hipEventSynchronize(ctx->ev1));
+    if (ctx->last_static) VKR_HIP(hipDeviceSynchronize());   // that call recorded no events
     DevCounters c{};
     const void* src = ctx->last_counters ? (const void*)ctx->last_counters : ctx->counters.p;
     VKR_HIP(hipMemcpy(&c, src, sizeof(c), hipMemcpyDeviceToHost));
@@ -440,6 +524,7 @@ vkr_error vkr_last_counters(vkr_ctx* ctx, vkr_counters* out)
 vkr_error vkr_last_kernel_ms(vkr_ctx* ctx, float* out_ms)
 {
     if (!ctx || !out_ms) return fail(VKR_INVALID_ARGUMENT, "NULL argument");
+    if (ctx->last_static) return fail(VKR_INVALID_ARGUMENT, "the last call (vkr_trace_frame_static) records no events");
     if (!ctx->timed) return fail(VKR_INVALID_ARGUMENT, "nothing launched yet");
     VKR_HIP(hipSetDevice(ctx->device));
     VKR_HIP(hipEventSynchronize(ctx->ev1));
@@ -530,6 +615,7 @@ vkr_error vkr_adjoint(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd, co
     VKR_HIP(launch_adjoint(a, s->dev.stack_needed, st));
     VKR_HIP(hipEventRecord(ctx->ev1, st));
     ctx->timed = true;
+    ctx->last_static = false;
     ctx->last_counters = nullptr;
     return VKR_SUCCESS;
 }

@@ -202,6 +202,7 @@ vkr_error vkr_trace_batch(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n_jo
     VKR_HIP(launch_path(a, v, grid, st));
     VKR_HIP(hipEventRecord(ctx->ev1, st));
     ctx->timed = true;
+    ctx->last_static = false;
     ctx->last_counters = nullptr;
     return VKR_SUCCESS;
 }
@@ -278,6 +279,7 @@ vkr_error vkr_graph_launch(vkr_ctx* ctx, uint32_t graph, void* stream)
     VKR_HIP(hipGraphLaunch(g.exec, st));
     VKR_HIP(hipEventRecord(ctx->ev1, st));
     ctx->timed = true;
+    ctx->last_static = false;
     ctx->last_counters = g.counters.as<DevCounters>();
     return VKR_SUCCESS;
 }

@@ -122,6 +122,7 @@ struct vkr_ctx {
     size_t host_jobs_bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_copy = nullptr;
     bool timed = false;
+    bool last_static = false;          // the last call was vkr_trace_frame_static (no events)
     const vkr::DevCounters* last_counters = nullptr;   // NULL = ctx counters
     std::vector<vkr::TuneEntry> tuned;
     std::vector<vkr::BatchTuneEntry> tuned_batch;

@@ -132,6 +132,16 @@ hipError_t launch_zero_counters(DevCounters* c, uint32_t* pool, hipStream_t s);
 hipError_t launch_set_frame(JobDesc* job, const vkr_frame_data& fd, hipStream_t s);
 // Occupancy-derived persistent grid size for (block, stack).
 uint32_t persistent_grid(const PathVariant& v, int device);
+
+// The static scratch of vkr_trace_frame_static: counters at offset 0, the
+// pool cursor at kStaticPoolOff, per-lane ray records from kStaticLanesOff.
+constexpr size_t kStaticPoolOff = 256;
+constexpr size_t kStaticLanesOff = 512;
+constexpr size_t kStaticScratchBytes = kStaticLanesOff + (size_t)VKR_STATIC_SCRATCH_LANE_BYTES;
+static_assert(sizeof(DevCounters) <= kStaticPoolOff, "counters fit below the pool cursor");
+// Device address of this device's static scratch (looked up once per device
+// with that device current, then cached).
+hipError_t static_scratch(int device, char** out);
 // Largest stack depth with a compiled kernel variant (0 when none fits).
 uint32_t stack_variant(uint32_t needed);
 

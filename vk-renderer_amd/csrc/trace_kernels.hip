@@ -22,6 +22,8 @@
 //     A window or list overflow is counted (documented near-tie).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "kernels.hpp"
 #include "traverse.hpp"
 #include "vkr_device.hpp"
@@ -1027,6 +1029,34 @@ hipError_t launch_set_frame(JobDesc* job, const vkr_frame_data& fd, hipStream_t 
 {
     hipLaunchKernelGGL(k_set_frame, dim3(1), dim3(64), 0, s, job, fd);
     return hipGetLastError();
+}
+
+// Scratch for callers that cannot allocate (vkr_trace_frame_static): one
+// array per device, held until the process ends.  Host code never passes
+// `scratch` itself to a kernel: its name is a host-side stand-in, and the
+// device address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) char scratch[kStaticScratchBytes];
+
+hipError_t static_scratch(int device, char** out)
+{
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static char* addr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        int cur = 0;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return e;
+        addr[device] = static_cast<char*>(p);
+    }
+    *out = addr[device];
+    return hipSuccess;
 }
 
 hipError_t launch_zero_counters(DevCounters* c, uint32_t* pool, hipStream_t s)

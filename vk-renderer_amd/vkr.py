@@ -155,6 +155,7 @@ def load(path: str = LIB_PATH):
         "vkr_scene_get_info": (e, [vp, u32, C.POINTER(SceneInfo)]),
         "vkr_scene_set_sphere_params": (e, [vp, u32, C.POINTER(C.c_float), u32]),
         "vkr_trace_frame": (e, [vp, u32, vp, C.POINTER(TraceParams), C.POINTER(Outputs), vp]),
+        "vkr_trace_frame_static": (e, [vp, u32, vp, C.POINTER(TraceParams), C.POINTER(Outputs), vp]),
         "vkr_trace_batch": (e, [vp, C.POINTER(SceneJob), u32, u32, u32, vp]),
         "vkr_batch_graph_create": (e, [vp, C.POINTER(SceneJob), u32, u32, u32, vp, C.POINTER(u32)]),
         "vkr_graph_launch": (e, [vp, u32, vp]),
@@ -329,6 +330,17 @@ class Context:
         st = _stream() if stream is None else stream
         check(self.lib.vkr_trace_frame(self.h, sid, fr.ctypes.data, C.byref(params), C.byref(out), st),
               "vkr_trace_frame")
+
+    def trace_static(self, sid: int, frame: np.ndarray, params: TraceParams, rgba8=None, radiance=None,
+                     prim_id=None, hit_t=None, ray_count=None, decisions=None, stream=None):
+        """vkr_trace_frame_static: the same frame without any allocation (the
+        kernels' scratch is the library's static device array)."""
+        fr = np.ascontiguousarray(np.asarray(frame, FRAME).reshape(1))
+        out = Outputs(_tptr(rgba8), _tptr(radiance), _tptr(prim_id), _tptr(hit_t), _tptr(ray_count),
+                      _tptr(decisions))
+        st = _stream() if stream is None else stream
+        check(self.lib.vkr_trace_frame_static(self.h, sid, fr.ctypes.data, C.byref(params), C.byref(out), st),
+              "vkr_trace_frame_static")
 
     def counters(self) -> dict:
         c = Counters()
