@@ -170,6 +170,8 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
     __shared__ uint32_t s_occ[BLOCK];                // per lane: occluded shadow rays (bit per slot)
     __shared__ uint2 s_res[BLOCK];                   // per lane: closest hit (prim, t)
     __shared__ uint32_t s_msk[BLOCK];                // per lane: slots holding a ray to trace
+    __shared__ uint32_t s_cpre[BLOCK / 64 * kRaySlots];            // per wave and slot: rays in lower slots
+    __shared__ unsigned long long s_colm[BLOCK / 64 * kRaySlots];  // per wave and slot: lanes holding it
     uint32_t* stk = s_stack + threadIdx.x;
     const uint32_t wbase = __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
     char* const wrec = A.lanes + ((size_t)blockIdx.x * BLOCK + wbase) * kLaneBytes;
@@ -435,23 +437,36 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
             if (phase == kPhTrace) tr = t;
             else occ = t.hit ? 1u : 0u;
         } else {
-            // The wave traces all queued rays together from its pool.
+            // The wave traces all queued rays together from its pool, slot
+            // by slot: column k holds slot k of every lane that queued one,
+            // lanes in order.  Neighbouring lanes then take rays of one slot
+            // (one emitter, or the path segments) from neighbouring pixels,
+            // which visit much the same nodes.
             const float4* park = reinterpret_cast<const float4*>(rec);
-            const unsigned long long lt = (1ull << __lane_id()) - 1ull;
-            uint32_t pre = 0, tot = 0;
-            for (uint32_t b = 0; (1u << b) <= kRaySlots; ++b) {   // the bits of nray (<= kRaySlots)
-                const unsigned long long m = __ballot((nray >> b) & 1u);
-                pre += (uint32_t)__popcll(m & lt) << b;
-                tot += (uint32_t)__popcll(m) << b;
+            const uint32_t mk = s_msk[threadIdx.x];
+            uint32_t tot = 0, my_pre = 0;
+            unsigned long long my_col = 0;
+#pragma unroll 1
+            for (uint32_t k = 0; k < kRaySlots; ++k) {
+                const unsigned long long c = __ballot((mk >> k) & 1u);
+                if (__lane_id() == k) {
+                    my_col = c;
+                    my_pre = tot;
+                }
+                tot += (uint32_t)__popcll(c);
+            }
+            const uint32_t wslot = (threadIdx.x >> 6) * kRaySlots;
+            if (__lane_id() < kRaySlots) {
+                s_cpre[wslot + __lane_id()] = my_pre;
+                s_colm[wslot + __lane_id()] = my_col;
             }
             s_occ[threadIdx.x] = 0u;
-            s_pre[threadIdx.x] = pre;
             // The pool, the records and the result words are shared by the
             // wave's lanes: order the writes above before the reads below.
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            traverse_pool<BLOCK, COUNT, BATCH>(BATCH ? A.jobs : &A.job, tot, s_pre + wbase, s_msk + wbase,
+            traverse_pool<BLOCK, COUNT, BATCH>(BATCH ? A.jobs : &A.job, tot, s_cpre + wslot, s_colm + wslot,
                                                s_occ + wbase, s_res + wbase, wrec, stk, wc);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();

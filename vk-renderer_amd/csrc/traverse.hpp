@@ -318,14 +318,15 @@ __device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, floa
 constexpr uint32_t kNone = 0xFFFFu;
 
 // Traces every ray the lanes of a wave queued for this step.  Lanes take rays
-// 0..total of the wave (lane l queued rays pre[l] .. pre[l+1]-1, held in the
-// slots of the set bits of msk[l]) until none is left, so a step costs the wave's total work plus the last
-// rays' tail instead of its longest lane.  A shadow ray sets bit `slot` of
-// occ[owner] when occluded; the closest-hit ray of lane `owner` writes
+// 0..total of the wave until none is left, so a step costs the wave's total
+// work plus the last rays' tail instead of its longest lane.  Rays are
+// numbered slot by slot: rays cpre[k] .. cpre[k+1]-1 are slot k of the lanes
+// whose bit is set in colm[k], in lane order.  A shadow ray sets bit `slot`
+// of occ[owner] when occluded; the closest-hit ray of lane `owner` writes
 // (prim, t) to res[owner].  `wrec` is the record of the wave's lane 0.
 template <int BLOCK, bool COUNT, bool BATCH>
-__device__ __forceinline__ void traverse_pool(const JobDesc* jobs, uint32_t total, const uint32_t* pre,
-                                              const uint32_t* msk, uint32_t* occ, uint2* res,
+__device__ __forceinline__ void traverse_pool(const JobDesc* jobs, uint32_t total, const uint32_t* cpre,
+                                              const unsigned long long* colm, uint32_t* occ, uint2* res,
                                               const char* __restrict__ wrec, uint32_t* stk, WorkCnt& wc)
 {
     // The job (scene, epsilon) of the ray in flight: a batched wave can hold
@@ -353,21 +354,21 @@ __device__ __forceinline__ void traverse_pool(const JobDesc* jobs, uint32_t tota
             if (want != 0 && next < total) {
                 const uint32_t rank = (uint32_t)__popcll(want & ((1ull << __lane_id()) - 1ull));
                 if (cur == kNone && next + rank < total) {
-                    // Ray e belongs to the last lane whose first queued ray is <= e.
+                    // Ray e is in the last slot whose first ray is <= e
+                    // (empty slots share their successor's start) ...
                     const uint32_t e = next + rank;
-                    uint32_t lo = 0, plo = 0;
-                    for (uint32_t step = 32; step > 0; step >>= 1) {
-                        const uint32_t p = pre[lo + step];
-                        if (p <= e) { lo += step; plo = p; }
-                    }
-                    // Its (e - plo)-th ray is in the slot of that set bit of msk[lo].
-                    uint32_t m = msk[lo], rem = e - plo, slot = 0;
-                    for (uint32_t wdt = 16; wdt > 0; wdt >>= 1) {
-                        const uint32_t c = (uint32_t)__popc(m & ((1u << wdt) - 1u));
+                    uint32_t slot = 0;
+                    for (uint32_t step = kRaySlots / 2; step > 0; step >>= 1)
+                        if (cpre[slot + step] <= e) slot += step;
+                    // ... held by the lane of the (e - cpre[slot])-th set bit of colm[slot].
+                    unsigned long long m = colm[slot];
+                    uint32_t rem = e - cpre[slot], lo = 0;
+                    for (uint32_t wdt = 32; wdt > 0; wdt >>= 1) {
+                        const uint32_t c = (uint32_t)__popcll(m & ((1ull << wdt) - 1ull));
                         if (rem >= c) {
                             rem -= c;
                             m >>= wdt;
-                            slot += wdt;
+                            lo += wdt;
                         }
                     }
                     cur = (lo << 5) | slot;
