@@ -80,12 +80,14 @@ constexpr uint32_t kMaxPlainLanes = 1u << 21;
 //              (metalness, v, bits of ei0, bits of the kNee* flags)
 //   kRecSalt   kRaySlots floats: the random stream position before each
 //              emitter's sample
+//   kRecPix    the sum of the pixel's finished samples (xyz) while it is traced
 constexpr uint32_t kRaySlots = 32;
 constexpr uint32_t kRecHead = 64u;
 constexpr uint32_t kRecSlots = 80u;
 constexpr uint32_t kRecHit = kRecSlots + 32u * kRaySlots;
 constexpr uint32_t kRecSalt = kRecHit + 48u;
-constexpr uint32_t kLaneBytes = kRecSalt + 4u * kRaySlots;
+constexpr uint32_t kRecPix = kRecSalt + 4u * kRaySlots;
+constexpr uint32_t kLaneBytes = kRecPix + 16u;
 
 struct PathLaunch {
     JobDesc job;                  // single-frame launches

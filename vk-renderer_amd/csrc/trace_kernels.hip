@@ -187,8 +187,7 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
     f3 P{}, N{}, dj{}, e{};
     const vkr_material* hm = nullptr;   // material of the current diffuse hit
     uint32_t ei = 0;
-    uint32_t rays_px = 0, prim0 = kMiss, drec = 0, drep = 0;
-    float t0 = 0.0f;
+    uint32_t rays_px = 0, drec = 0, drep = 0;
     uint32_t c_rays = 0, c_shadow = 0, c_traced = 0, wave_steps = 0;
     bool new_path = false;
     uint32_t nee_flags = 0;             // kNee* of the current diffuse hit
@@ -219,10 +218,11 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
         uvy = (float)y / (float)j.H;
         salt = uvy / uvx + j.seed;                   // :565-566
         smp = 0;
-        accum = f3{0.0f, 0.0f, 0.0f};
+        // The sum of finished samples: in the lane's record with the ray
+        // pool (fewer registers live across the step), else in registers.
+        if constexpr (POOL) *reinterpret_cast<float4*>(rec + kRecPix) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        else accum = f3{0.0f, 0.0f, 0.0f};
         rays_px = 0;
-        prim0 = kMiss;
-        t0 = 0.0f;
         new_path = true;
     };
 
@@ -490,7 +490,11 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
 
         bool fin_diffuse = false, do_rr = false, path_end = false;
         if (phase == kPhTrace) {
-            if (smp == 0 && depth == 0) { prim0 = tr.prim; t0 = tr.t; }
+            if (smp == 0 && depth == 0) {                            // the primary hit, written at once
+                const size_t pi = (size_t)py * j.W + px;
+                if (j.prim_id) j.prim_id[pi] = tr.prim;
+                if (j.hit_t) j.hit_t[pi] = tr.t;
+            }
             if (!tr.hit) {
                 path_end = true;                                     // :445
             } else {
@@ -631,7 +635,14 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
         if (path_end) {                                              // :578-581
             if ((j.flags & VKR_TRACE_RECORD_DECISIONS) && j.decisions)
                 j.decisions[((size_t)py * j.W + px) * j.spp + smp] = (uint16_t)drec;
-            accum = add(accum, acc);
+            if constexpr (POOL) {
+                float4* ap = reinterpret_cast<float4*>(rec + kRecPix);
+                const float4 a = *ap;
+                accum = add(f3{a.x, a.y, a.z}, acc);
+                *ap = make_float4(accum.x, accum.y, accum.z, 0.0f);
+            } else {
+                accum = add(accum, acc);
+            }
             ++smp;
             if (smp < j.spp) {
                 new_path = true;
@@ -646,8 +657,6 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
                 if (j.rgba8)
                     reinterpret_cast<uint32_t*>(j.rgba8)[pi] =
                         unorm8(c.x) | (unorm8(c.y) << 8) | (unorm8(c.z) << 16) | (255u << 24);
-                if (j.prim_id) j.prim_id[pi] = prim0;
-                if (j.hit_t) j.hit_t[pi] = t0;
                 if (j.ray_count) j.ray_count[pi] = rays_px;
                 has = false;
             }

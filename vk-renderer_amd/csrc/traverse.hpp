@@ -65,7 +65,7 @@ __device__ __forceinline__ void cswap(float& ta, uint32_t& ka, float& tb, uint32
 struct Ray {
     f3 o, d;
     float tmax, tmax_eps, cull, tmin;
-    float ix, iy, iz, ox, oy, oz, ax, ay, az, px, py, pz;
+    float ix, iy, iz, ox, oy, oz, pad;   // |ix| etc. come free as source modifiers
     float ct0, ct1, ct2, ct3;
     uint32_t ck0, ck1, ck2, ck3;
     bool any, ovf, occluded, nan_ray;
@@ -93,19 +93,13 @@ struct Ray {
         ox = o.x * ix;
         oy = o.y * iy;
         oz = o.z * iz;
-        ax = fabsf(ix);
-        ay = fabsf(iy);
-        az = fabsf(iz);
         // The sphere test (sphere_t, Assets/Tracer.comp:314-330) rounds its
         // discriminant with an error up to ~60 ulp of |o - c|^2, so from far
         // away it reports hits up to ~1e-3 |o - c| outside the sphere.  The
         // sphere subtree's boxes are widened by 2e-3 (|o - C| + R), which
         // bounds that distance for every centre c within R of C.
         const f3 oc = sub(o, f3{sc.sph_c[0], sc.sph_c[1], sc.sph_c[2]});
-        const float pad = 2e-3f * (sqrtf(dot(oc, oc)) + sc.sph_r);
-        px = fminf(pad * ax, 3e38f);
-        py = fminf(pad * ay, 3e38f);
-        pz = fminf(pad * az, 3e38f);
+        pad = fminf(2e-3f * (sqrtf(dot(oc, oc)) + sc.sph_r), 3e38f);
         ref = (sc.has_bvh && !nan_ray) ? 0u : kDone;
         sp = 0;
     }
@@ -126,8 +120,9 @@ struct Ray {
         const float cb = cull + fabsf(cull) * 1e-5f + 1e-5f;
         // Slab test on centre/half-extent boxes: t = (c - o) i -+ (e + w) |i|,
         // with the widening w only in the sphere subtree (m = 1 there).
-        const float m = __uint_as_float(D.z);
-        const float qx = m * px, qy = m * py, qz = m * pz;
+        const float mp = __uint_as_float(D.z) * pad;
+        const float ax = fabsf(ix), ay = fabsf(iy), az = fabsf(iz);
+        const float qx = mp * ax, qy = mp * ay, qz = mp * az;
         const float a0 = __builtin_fmaf(A.x, ix, -ox), a1 = __builtin_fmaf(A.y, ax, qx);
         const float a2 = __builtin_fmaf(A.z, iy, -oy), a3 = __builtin_fmaf(A.w, ay, qy);
         const float a4 = __builtin_fmaf(B.x, iz, -oz), a5 = __builtin_fmaf(B.y, az, qz);
