@@ -66,28 +66,36 @@ constexpr uint32_t kDefaultChunk = 8u;
 // bounds their ray records (2^21 x kLaneBytes = 2.7 GB).
 constexpr uint32_t kMaxPlainLanes = 1u << 21;
 
-// Per-lane ray record of the path kernel (one per lane of a launch, in
-// global memory):
-//   [0, 64)    path state parked while the wave traces
-//   [64, 80)   this step's ray origin xyz; w holds the bits of
-//              (job << 1) | 1 for shadow rays, (job << 1) for a closest-hit
-//              ray (job: index in the batch, 0 for a single frame)
-//   [80, ...)  kRaySlots slots of 32 bytes: (d xyz, t_max), (next-event
-//              contribution xyz, 0); slot k holds emitter ei0 + k of a
-//              next-event window, or slot 0 the path segment
-//   kRecHit    the diffuse hit published for the wave's next-event
-//              evaluation: (N xyz, u), (albedo xyz, roughness),
-//              (metalness, v, bits of ei0, bits of the kNee* flags)
-//   kRecSalt   kRaySlots floats: the random stream position before each
-//              emitter's sample
-//   kRecPix    the sum of the pixel's finished samples (xyz) while it is traced
+// Ray records of the pooled path kernel, in global memory: one block of
+// kWaveBytes per wave (64 lanes), laid out field by field so that the lanes
+// of a wave reading or writing one field touch contiguous addresses.  Field
+// f of lane l is at f + 16 l; per-slot fields hold slot k of lane l at
+// f + 16 (64 k + l) (salts: f + 4 (64 k + l)).
+//   kWPark   4 x float4: path state parked while the wave traces
+//            (ro, dj.x) (rd, dj.y) (P, dj.z) (N, 0), at kWPark + 1024 i
+//   kWHead   this step's ray origin xyz; w holds the bits of (job << 1) | 1
+//            for shadow rays, (job << 1) for a closest-hit ray (job: index
+//            in the batch, 0 for a single frame)
+//   kWSlotD  per slot: the ray's (d xyz, t_max); slot k holds emitter
+//            ei0 + k of a next-event window, or slot 0 the path segment
+//   kWSlotC  per slot: the next-event contribution (xyz, 0)
+//   kWHit    3 x float4: the diffuse hit published for the wave's
+//            next-event evaluation: (N xyz, u), (albedo xyz, roughness),
+//            (metalness, v, bits of ei0, bits of the kNee* flags)
+//   kWSalt   per slot: the random stream position before emitter ei0 + k's
+//            sample
+//   kWPix    the sum of the pixel's finished samples (xyz)
 constexpr uint32_t kRaySlots = 32;
-constexpr uint32_t kRecHead = 64u;
-constexpr uint32_t kRecSlots = 80u;
-constexpr uint32_t kRecHit = kRecSlots + 32u * kRaySlots;
-constexpr uint32_t kRecSalt = kRecHit + 48u;
-constexpr uint32_t kRecPix = kRecSalt + 4u * kRaySlots;
-constexpr uint32_t kLaneBytes = kRecPix + 16u;
+constexpr uint32_t kWPark = 0u;
+constexpr uint32_t kWHead = kWPark + 4u * 1024u;
+constexpr uint32_t kWSlotD = kWHead + 1024u;
+constexpr uint32_t kWSlotC = kWSlotD + kRaySlots * 1024u;
+constexpr uint32_t kWHit = kWSlotC + kRaySlots * 1024u;
+constexpr uint32_t kWSalt = kWHit + 3u * 1024u;
+constexpr uint32_t kWPix = kWSalt + kRaySlots * 256u;
+constexpr uint32_t kWaveBytes = kWPix + 1024u;
+constexpr uint32_t kLaneBytes = kWaveBytes / 64u;   // what one lane of a launch needs
+static_assert(kWaveBytes % 64u == 0u, "a wave's records split evenly over its lanes");
 
 struct PathLaunch {
     JobDesc job;                  // single-frame launches
@@ -100,7 +108,7 @@ struct PathLaunch {
     uint32_t chunk;               // refill when at least this many lanes idle
     uint32_t* pool;               // persistent pool cursor (device)
     DevCounters* counters;
-    char* lanes;                  // per-lane ray records, grid x block x kLaneBytes (traverse.hpp)
+    char* lanes;                  // ray records: one kWaveBytes block per wave of the grid
 };
 
 // Adjoint launch (path replay), see adjoint.hip.

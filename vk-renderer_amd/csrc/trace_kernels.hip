@@ -166,16 +166,18 @@ template <int BLOCK, int STACK, bool COUNT, bool BATCH, bool POOL>
 __device__ __forceinline__ void path_program(const PathLaunch& A)
 {
     __shared__ uint32_t s_stack[STACK * BLOCK];
-    __shared__ uint32_t s_pre[BLOCK];                // per lane: index of its first queued ray in the wave
     __shared__ uint32_t s_occ[BLOCK];                // per lane: occluded shadow rays (bit per slot)
     __shared__ uint2 s_res[BLOCK];                   // per lane: closest hit (prim, t)
     __shared__ uint32_t s_msk[BLOCK];                // per lane: slots holding a ray to trace
-    __shared__ uint32_t s_cpre[BLOCK / 64 * kRaySlots];            // per wave and slot: rays in lower slots
-    __shared__ unsigned long long s_colm[BLOCK / 64 * kRaySlots];  // per wave and slot: lanes holding it
+    __shared__ uint32_t s_cpre[BLOCK / 64 * kRaySlots];            // per wave and slot: items in lower slots
+    __shared__ unsigned long long s_colm[BLOCK / 64 * kRaySlots];  // per wave and slot: lanes holding one
     uint32_t* stk = s_stack + threadIdx.x;
     const uint32_t wbase = __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
-    char* const wrec = A.lanes + ((size_t)blockIdx.x * BLOCK + wbase) * kLaneBytes;
-    char* const rec = wrec + __lane_id() * kLaneBytes;
+    const uint32_t wslot = (wbase >> 6) * kRaySlots;
+    char* const wrec = A.lanes + ((size_t)blockIdx.x * (BLOCK / 64) + (wbase >> 6)) * kWaveBytes;
+    // This lane's element of a one-per-lane field f is at wrec + (f + lo16):
+    // a wave-uniform base and a 32-bit offset, one address form for all fields.
+    const uint32_t lo16 = __lane_id() * 16u;
 
     // Lane state.
     bool has = false;
@@ -220,7 +222,7 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
         smp = 0;
         // The sum of finished samples: in the lane's record with the ray
         // pool (fewer registers live across the step), else in registers.
-        if constexpr (POOL) *reinterpret_cast<float4*>(rec + kRecPix) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if constexpr (POOL) *reinterpret_cast<float4*>(wrec + (kWPix + lo16)) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         else accum = f3{0.0f, 0.0f, 0.0f};
         rays_px = 0;
         new_path = true;
@@ -290,9 +292,9 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
                 const float dp1 = (float)(depth + 1u);
                 const float tmax = 3000.0f / (dp1 * dp1);            // :444
                 if constexpr (POOL) {
-                    *reinterpret_cast<float4*>(rec + kRecHead) =
+                    *reinterpret_cast<float4*>(wrec + (kWHead + lo16)) =
                         make_float4(ro.x, ro.y, ro.z, __uint_as_float(BATCH ? job << 1 : 0u));
-                    *reinterpret_cast<float4*>(rec + kRecSlots) = make_float4(rd.x, rd.y, rd.z, tmax);
+                    *reinterpret_cast<float4*>(wrec + (kWSlotD + lo16)) = make_float4(rd.x, rd.y, rd.z, tmax);
                 } else {
                     o0 = ro;
                     d0 = rd;
@@ -322,14 +324,14 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
                     // together below.  Each sample draws twice (:464-465), so
                     // the stream position before each one is known now.
                     const uint32_t w = min(kRaySlots, j.sc.n_emitters - ei);
-                    float4* hr = reinterpret_cast<float4*>(rec + kRecHit);
-                    hr[0] = make_float4(N.x, N.y, N.z, uvx);
-                    hr[1] = make_float4(h.alb.x, h.alb.y, h.alb.z, h.rough);
-                    hr[2] = make_float4(h.metal, uvy, __uint_as_float(ei), __uint_as_float(nee_flags));
-                    float* sk = reinterpret_cast<float*>(rec + kRecSalt);
+                    *reinterpret_cast<float4*>(wrec + (kWHit + lo16)) = make_float4(N.x, N.y, N.z, uvx);
+                    *reinterpret_cast<float4*>(wrec + (kWHit + 1024u + lo16)) = make_float4(h.alb.x, h.alb.y, h.alb.z, h.rough);
+                    *reinterpret_cast<float4*>(wrec + (kWHit + 2048u + lo16)) =
+                        make_float4(h.metal, uvy, __uint_as_float(ei), __uint_as_float(nee_flags));
+                    float* sk = reinterpret_cast<float*>(wrec + (kWSalt + __lane_id() * 4u));
                     float sv = salt;
                     for (uint32_t k = 0; k < w; ++k) {
-                        sk[k] = sv;
+                        sk[k * 64u] = sv;
                         sv = sv + 1.0f;
                         sv = sv + 1.0f;
                     }
@@ -339,7 +341,7 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
                     c_shadow += w;
                     ei += w;
                     nitem = w;
-                    *reinterpret_cast<float4*>(rec + kRecHead) =
+                    *reinterpret_cast<float4*>(wrec + (kWHead + lo16)) =
                         make_float4(P.x, P.y, P.z, __uint_as_float(BATCH ? (job << 1) | 1u : 1u));
                 } else {
                     while (ei < j.sc.n_emitters) {
@@ -363,23 +365,17 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
         if constexpr (POOL) {
             // Path state that the wave's evaluation and tracing do not need
             // waits in the record, so it holds no registers meanwhile.
-            float4* park = reinterpret_cast<float4*>(rec);
+            float4* park = reinterpret_cast<float4*>(wrec + (kWPark + lo16));
             park[0] = make_float4(ro.x, ro.y, ro.z, dj.x);
-            park[1] = make_float4(rd.x, rd.y, rd.z, dj.y);
-            park[2] = make_float4(P.x, P.y, P.z, dj.z);
-            park[3] = make_float4(N.x, N.y, N.z, 0.0f);
-            // The wave evaluates every published (lane, emitter) pair: lane l's
-            // pairs are items s_pre[l] .. s_pre[l + 1] - 1.  A nonzero
-            // contribution goes to slot k of the owner's record (emitter
-            // ei0 + k) and sets bit k of its mask.
-            const unsigned long long lt = (1ull << __lane_id()) - 1ull;
-            uint32_t pre = 0, tot = 0;
-            for (uint32_t b = 0; (1u << b) <= kRaySlots; ++b) {   // the bits of nitem (<= kRaySlots)
-                const unsigned long long m = __ballot((nitem >> b) & 1u);
-                pre += (uint32_t)__popcll(m & lt) << b;
-                tot += (uint32_t)__popcll(m) << b;
-            }
-            s_pre[threadIdx.x] = pre;
+            park[64] = make_float4(rd.x, rd.y, rd.z, dj.y);
+            park[128] = make_float4(P.x, P.y, P.z, dj.z);
+            park[192] = make_float4(N.x, N.y, N.z, 0.0f);
+            // The wave evaluates every published (lane, emitter) pair, slot
+            // by slot (neighbouring lanes take one emitter of neighbouring
+            // owners, whose records are neighbours).  A nonzero contribution
+            // goes to slot k of the owner (emitter ei0 + k) and sets bit k
+            // of its mask.
+            const uint32_t tot = pool_columns(nitem >= 32u ? ~0u : (1u << nitem) - 1u, s_cpre + wslot, s_colm + wslot);
             s_msk[threadIdx.x] = 0u;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -387,18 +383,14 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
             for (uint32_t base = 0; base < tot; base += 64u) {
                 const uint32_t e = base + __lane_id();
                 if (e < tot) {
-                    uint32_t lo = 0, plo = 0;
-                    for (uint32_t step = 32; step > 0; step >>= 1) {
-                        const uint32_t p = s_pre[wbase + lo + step];
-                        if (p <= e) { lo += step; plo = p; }
-                    }
-                    const uint32_t k = e - plo;
-                    const char* orec = wrec + lo * kLaneBytes;
-                    const float4 hd = *reinterpret_cast<const float4*>(orec + kRecHead);
-                    const float4 h0 = reinterpret_cast<const float4*>(orec + kRecHit)[0];
-                    const float4 h1 = reinterpret_cast<const float4*>(orec + kRecHit)[1];
-                    const float4 h2 = reinterpret_cast<const float4*>(orec + kRecHit)[2];
-                    const float sk = reinterpret_cast<const float*>(orec + kRecSalt)[k];
+                    uint32_t k, lo;
+                    pool_item(s_cpre + wslot, s_colm + wslot, e, k, lo);
+                    const uint32_t oo = lo * 16u;
+                    const float4 hd = *reinterpret_cast<const float4*>(wrec + (kWHead + oo));
+                    const float4 h0 = *reinterpret_cast<const float4*>(wrec + (kWHit + oo));
+                    const float4 h1 = *reinterpret_cast<const float4*>(wrec + (kWHit + 1024u + oo));
+                    const float4 h2 = *reinterpret_cast<const float4*>(wrec + (kWHit + 2048u + oo));
+                    const float sk = *reinterpret_cast<const float*>(wrec + (kWSalt + (k * 64u + lo) * 4u));
                     const JobDesc& oj = BATCH ? A.jobs[__float_as_uint(hd.w) >> 1] : A.job;
                     NeeHit h;
                     h.P = f3{hd.x, hd.y, hd.z};
@@ -411,9 +403,9 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
                     f3 L, c;
                     float tl;
                     if (nee_sample(oj, h, __float_as_uint(h2.z) + k, sk, h0.w, h2.y, L, tl, c)) {
-                        float4* sl = reinterpret_cast<float4*>(const_cast<char*>(orec) + kRecSlots + k * 32u);
-                        sl[0] = make_float4(L.x, L.y, L.z, tl);
-                        sl[1] = make_float4(c.x, c.y, c.z, 0.0f);
+                        const uint32_t so = (k * 64u + lo) * 16u;
+                        *reinterpret_cast<float4*>(wrec + (kWSlotD + so)) = make_float4(L.x, L.y, L.z, tl);
+                        *reinterpret_cast<float4*>(wrec + (kWSlotC + so)) = make_float4(c.x, c.y, c.z, 0.0f);
                         atomicOr(s_msk + wbase + lo, 1u << k);
                     }
                 }
@@ -442,24 +434,8 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
             // lanes in order.  Neighbouring lanes then take rays of one slot
             // (one emitter, or the path segments) from neighbouring pixels,
             // which visit much the same nodes.
-            const float4* park = reinterpret_cast<const float4*>(rec);
-            const uint32_t mk = s_msk[threadIdx.x];
-            uint32_t tot = 0, my_pre = 0;
-            unsigned long long my_col = 0;
-#pragma unroll 1
-            for (uint32_t k = 0; k < kRaySlots; ++k) {
-                const unsigned long long c = __ballot((mk >> k) & 1u);
-                if (__lane_id() == k) {
-                    my_col = c;
-                    my_pre = tot;
-                }
-                tot += (uint32_t)__popcll(c);
-            }
-            const uint32_t wslot = (threadIdx.x >> 6) * kRaySlots;
-            if (__lane_id() < kRaySlots) {
-                s_cpre[wslot + __lane_id()] = my_pre;
-                s_colm[wslot + __lane_id()] = my_col;
-            }
+            const float4* park = reinterpret_cast<const float4*>(wrec + (kWPark + lo16));
+            const uint32_t tot = pool_columns(s_msk[threadIdx.x], s_cpre + wslot, s_colm + wslot);
             s_occ[threadIdx.x] = 0u;
             // The pool, the records and the result words are shared by the
             // wave's lanes: order the writes above before the reads below.
@@ -471,7 +447,7 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const float4 k0 = park[0], k1 = park[1], k2 = park[2], k3 = park[3];
+            const float4 k0 = park[0], k1 = park[64], k2 = park[128], k3 = park[192];
             ro = f3{k0.x, k0.y, k0.z};
             rd = f3{k1.x, k1.y, k1.z};
             P = f3{k2.x, k2.y, k2.z};
@@ -594,7 +570,7 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
                 while (bits) {
                     const uint32_t k = (uint32_t)__builtin_ctz(bits);
                     bits &= bits - 1u;
-                    const float4 c = reinterpret_cast<const float4*>(rec + kRecSlots + k * 32u)[1];
+                    const float4 c = *reinterpret_cast<const float4*>(wrec + (kWSlotC + k * 1024u + lo16));
                     e = add(e, f3{c.x, c.y, c.z});                   // :502
                 }
             } else if (nray && !(occ & 1u)) {
@@ -636,7 +612,7 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
             if ((j.flags & VKR_TRACE_RECORD_DECISIONS) && j.decisions)
                 j.decisions[((size_t)py * j.W + px) * j.spp + smp] = (uint16_t)drec;
             if constexpr (POOL) {
-                float4* ap = reinterpret_cast<float4*>(rec + kRecPix);
+                float4* ap = reinterpret_cast<float4*>(wrec + (kWPix + lo16));
                 const float4 a = *ap;
                 accum = add(f3{a.x, a.y, a.z}, acc);
                 *ap = make_float4(accum.x, accum.y, accum.z, 0.0f);

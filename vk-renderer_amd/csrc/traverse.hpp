@@ -312,13 +312,66 @@ __device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, floa
 
 constexpr uint32_t kNone = 0xFFFFu;
 
+// Numbers a wave's work items slot by slot: `bits` holds the slots this lane
+// has an item in.  Items cpre[k] .. cpre[k+1]-1 are slot k of the lanes
+// whose bit is set in colm[k], in lane order; slots past the last occupied
+// one start at the total.  Returns the wave's total (wave-uniform).  The
+// caller orders these writes before the lookups (pool_item).
+__device__ __forceinline__ uint32_t pool_columns(uint32_t bits, uint32_t* cpre, unsigned long long* colm)
+{
+    uint32_t tot = 0, my_pre = 0;
+    unsigned long long my_col = 0;
+    uint32_t k = 0;
+#pragma unroll 1
+    for (; k < kRaySlots; ++k) {
+        if (__ballot((bits >> k) != 0u) == 0) break;   // no lane holds slot k or above
+        const unsigned long long c = __ballot((bits >> k) & 1u);
+        if (__lane_id() == k) {
+            my_col = c;
+            my_pre = tot;
+        }
+        tot += (uint32_t)__popcll(c);
+    }
+    if (__lane_id() >= k) my_pre = tot;
+    if (__lane_id() < kRaySlots) {
+        cpre[__lane_id()] = my_pre;
+        colm[__lane_id()] = my_col;
+    }
+    return tot;
+}
+
+// Item e (< the total of pool_columns) is slot `slot` of lane `lane`.
+__device__ __forceinline__ void pool_item(const uint32_t* cpre, const unsigned long long* colm, uint32_t e,
+                                          uint32_t& slot, uint32_t& lane)
+{
+    // Item e is in the last slot whose first item is <= e (empty slots share
+    // their successor's start) ...
+    slot = 0;
+    for (uint32_t step = kRaySlots / 2; step > 0; step >>= 1)
+        if (cpre[slot + step] <= e) slot += step;
+    // ... held by the lane of the (e - cpre[slot])-th set bit of colm[slot].
+    unsigned long long m = colm[slot];
+    uint32_t rem = e - cpre[slot], lo = 0;
+    for (uint32_t wdt = 32; wdt > 0; wdt >>= 1) {
+        const uint32_t c = (uint32_t)__popcll(m & ((1ull << wdt) - 1ull));
+        if (rem >= c) {
+            rem -= c;
+            m >>= wdt;
+            lo += wdt;
+        }
+    }
+    lane = lo;
+}
+
 // Traces every ray the lanes of a wave queued for this step.  Lanes take rays
 // 0..total of the wave until none is left, so a step costs the wave's total
 // work plus the last rays' tail instead of its longest lane.  Rays are
 // numbered slot by slot: rays cpre[k] .. cpre[k+1]-1 are slot k of the lanes
 // whose bit is set in colm[k], in lane order.  A shadow ray sets bit `slot`
 // of occ[owner] when occluded; the closest-hit ray of lane `owner` writes
-// (prim, t) to res[owner].  `wrec` is the record of the wave's lane 0.
+// (prim, t) to res[owner].  `wrec` is the wave's block of ray records
+// (kWHead, kWSlotD: the lanes of a wave reading one slot of neighbouring
+// owners read neighbouring addresses).
 template <int BLOCK, bool COUNT, bool BATCH>
 __device__ __forceinline__ void traverse_pool(const JobDesc* jobs, uint32_t total, const uint32_t* cpre,
                                               const unsigned long long* colm, uint32_t* occ, uint2* res,
@@ -349,27 +402,11 @@ __device__ __forceinline__ void traverse_pool(const JobDesc* jobs, uint32_t tota
             if (want != 0 && next < total) {
                 const uint32_t rank = (uint32_t)__popcll(want & ((1ull << __lane_id()) - 1ull));
                 if (cur == kNone && next + rank < total) {
-                    // Ray e is in the last slot whose first ray is <= e
-                    // (empty slots share their successor's start) ...
-                    const uint32_t e = next + rank;
-                    uint32_t slot = 0;
-                    for (uint32_t step = kRaySlots / 2; step > 0; step >>= 1)
-                        if (cpre[slot + step] <= e) slot += step;
-                    // ... held by the lane of the (e - cpre[slot])-th set bit of colm[slot].
-                    unsigned long long m = colm[slot];
-                    uint32_t rem = e - cpre[slot], lo = 0;
-                    for (uint32_t wdt = 32; wdt > 0; wdt >>= 1) {
-                        const uint32_t c = (uint32_t)__popcll(m & ((1ull << wdt) - 1ull));
-                        if (rem >= c) {
-                            rem -= c;
-                            m >>= wdt;
-                            lo += wdt;
-                        }
-                    }
+                    uint32_t slot, lo;
+                    pool_item(cpre, colm, next + rank, slot, lo);
                     cur = (lo << 5) | slot;
-                    const char* rec = wrec + (cur >> 5) * kLaneBytes;
-                    const float4 h = *reinterpret_cast<const float4*>(rec + kRecHead);
-                    const float4 s = *reinterpret_cast<const float4*>(rec + kRecSlots + (cur & 31u) * 32u);
+                    const float4 h = *reinterpret_cast<const float4*>(wrec + (kWHead + lo * 16u));
+                    const float4 s = *reinterpret_cast<const float4*>(wrec + (kWSlotD + (slot * 64u + lo) * 16u));
                     const uint32_t hw = __float_as_uint(h.w);   // (owner's job << 1) | shadow ray
                     if constexpr (BATCH) rj = jobs + (hw >> 1);
                     r.start(rj->sc, f3{h.x, h.y, h.z}, f3{s.x, s.y, s.z}, s.w, rj->eps, (hw & 1u) != 0u);
