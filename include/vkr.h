@@ -344,7 +344,7 @@ vkr_error vkr_autotune(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* frame
  * Assets/Tracer.comp:142 and Source/GraphicsDevice.cpp:1266).  Times
  * vkr_trace_batch on these jobs (they write their outputs) by coordinate
  * descent over workgroup size (64, 128, 256), pool chunk (1..32) and stack
- * depth (32, 64 when the scenes allow 32), `repeats` timed launches per
+ * depth (32 and 64 above what the scenes need), `repeats` timed launches per
  * candidate, and remembers the fastest for later vkr_trace_batch and
  * vkr_batch_graph_create calls on the same job set (scenes, image sizes,
  * spp, depth and flags, in order) that pass block = chunk = 0.  Results do

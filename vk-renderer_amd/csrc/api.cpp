@@ -347,6 +347,21 @@ vkr_error vkr_scene_set_sphere_params(vkr_ctx* ctx, uint32_t scene, const float*
     return VKR_SUCCESS;
 }
 
+// Trees of depth <= 24 can take the 24-entry-stack kernel, which keeps 5
+// waves per SIMD resident (4 with deeper stacks) but spills more in its
+// per-step code.  The extra wave pays off only when the image has well over
+// the pixels the 4-wave kernel keeps resident (1080p soup frame: 90 ms
+// against 95 ms; a 512 x 512 frame, which the 4-wave grid holds at once, 7%
+// slower), so untuned smaller frames take the 32-entry kernel.
+static uint32_t default_stack(const PathVariant& v, uint32_t total, int device)
+{
+    if (v.stack != 24u) return v.stack;
+    PathVariant v4 = v;
+    v4.stack = 32u;
+    const uint64_t resident4 = (uint64_t)persistent_grid(v4, device) * v.block;
+    return (uint64_t)total > 2u * resident4 ? 24u : 32u;
+}
+
 vkr_error vkr_trace_frame(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd, const vkr_trace_params* params,
                           const vkr_outputs* outputs, void* stream)
 {
@@ -383,7 +398,8 @@ vkr_error vkr_trace_frame(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd
         a.counters = cnt;
         uint32_t block = params->block, chunk = params->chunk, sched = params->sched;
         uint32_t stack = s->dev.stack_needed;
-        if (const TuneEntry* t = find_tuned(ctx, scene, *params)) {
+        const TuneEntry* tuned = find_tuned(ctx, scene, *params);
+        if (const TuneEntry* t = tuned) {
             if (!block) block = t->block;
             if (!chunk) chunk = t->chunk;
             if (sched == VKR_SCHED_AUTO) sched = t->sched;
@@ -393,8 +409,9 @@ vkr_error vkr_trace_frame(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd
         const bool persistent = sched != VKR_SCHED_PLAIN;
         a.persistent = persistent ? 1u : 0u;
         a.chunk = chunk ? chunk : kDefaultChunk;
-        const PathVariant v{block, stack, (ctx->flags & VKR_CTX_COUNT_WORK) != 0, false,
-                            use_ray_pool(params->flags, s->dev.n_emitters)};
+        PathVariant v{block, stack, (ctx->flags & VKR_CTX_COUNT_WORK) != 0, false,
+                      use_ray_pool(params->flags, s->dev.n_emitters)};
+        if (!tuned) v.stack = default_stack(v, a.total, ctx->device);
         // Persistent: one resident grid.  One pixel per lane: launches of at
         // most kMaxPlainLanes lanes each, in pool order.
         const uint32_t step = persistent ? a.total : kMaxPlainLanes;
@@ -457,7 +474,8 @@ vkr_error vkr_trace_frame_static(vkr_ctx* ctx, uint32_t scene, const vkr_frame_d
         a.counters = cnt;
         uint32_t block = params->block, chunk = params->chunk, sched = params->sched;
         uint32_t stack = s->dev.stack_needed;
-        if (const TuneEntry* t = find_tuned(ctx, scene, *params)) {
+        const TuneEntry* tuned = find_tuned(ctx, scene, *params);
+        if (const TuneEntry* t = tuned) {
             if (!block) block = t->block;
             if (!chunk) chunk = t->chunk;
             if (sched == VKR_SCHED_AUTO) sched = t->sched;
@@ -467,8 +485,17 @@ vkr_error vkr_trace_frame_static(vkr_ctx* ctx, uint32_t scene, const vkr_frame_d
         const bool persistent = sched != VKR_SCHED_PLAIN;
         a.persistent = persistent ? 1u : 0u;
         a.chunk = chunk ? chunk : kDefaultChunk;
-        const PathVariant v{block, stack, (ctx->flags & VKR_CTX_COUNT_WORK) != 0, false,
-                            use_ray_pool(params->flags, s->dev.n_emitters)};
+        PathVariant v{block, stack, (ctx->flags & VKR_CTX_COUNT_WORK) != 0, false,
+                      use_ray_pool(params->flags, s->dev.n_emitters)};
+        if (!tuned) {
+            v.stack = default_stack(v, a.total, ctx->device);
+            // The static records cap the pooled grid below what even 4 waves
+            // keep resident; then the fifth wave cannot help.
+            PathVariant v4 = v;
+            v4.stack = 32u;
+            if (v.stack == 24u && v.pool && max_lanes <= (size_t)persistent_grid(v4, ctx->device) * block)
+                v.stack = 32u;
+        }
         const uint32_t step = persistent ? a.total : kMaxPlainLanes;
         uint32_t grid = persistent ? std::min(persistent_grid(v, ctx->device), (a.total + block - 1u) / block)
                                    : (std::min(a.total, step) + block - 1u) / block;

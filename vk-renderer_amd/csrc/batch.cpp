@@ -321,7 +321,7 @@ vkr_error vkr_autotune(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd, c
 
     struct Cand { uint32_t block, chunk, stack, sched; };
     std::vector<Cand> cands;
-    const uint32_t stacks[2] = {32u, 64u};
+    const uint32_t stacks[3] = {24u, 32u, 64u};
     for (uint32_t b : {64u, 128u, 256u}) cands.push_back({b, 0u, s->dev.stack_needed, (uint32_t)VKR_SCHED_PLAIN});
     for (uint32_t st : stacks) {
         if (st < s->dev.stack_needed) continue;

@@ -654,10 +654,14 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
     }
 }
 
-// The timed kernel (4 waves per SIMD: at most 128 VGPRs) and the work-counting
-// variant (same traversal order, more registers).
+// The timed kernel and the work-counting variant (same traversal order, more
+// registers).  The traversal waits on dependent node fetches, so resident
+// waves set the speed: with a 24-entry stack a 128-lane block needs 15.1 KB
+// of LDS, ten fit a CU, and the kernel is held to 96 VGPRs for 5 waves per
+// SIMD (its spills are all in per-step code, outside the pool's traversal
+// loops); with deeper stacks LDS allows 4 waves (128 VGPRs).
 template <int BLOCK, int STACK, bool BATCH, bool POOL>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_path(const PathLaunch A)
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(STACK <= 24 ? 5 : 4))) void k_path(const PathLaunch A)
 {
     path_program<BLOCK, STACK, false, BATCH, POOL>(A);
 }
@@ -949,14 +953,15 @@ const void* fn_flags(const PathVariant& v)
 
 const void* kernel_ptr(const PathVariant& v)
 {
-    if (v.block == 64) return v.stack <= 32 ? fn_flags<64, 32>(v) : fn_flags<64, 64>(v);
-    if (v.block == 128) return v.stack <= 32 ? fn_flags<128, 32>(v) : fn_flags<128, 64>(v);
-    return v.stack <= 32 ? fn_flags<256, 32>(v) : fn_flags<256, 64>(v);
+    if (v.block == 64) return v.stack <= 24 ? fn_flags<64, 24>(v) : v.stack <= 32 ? fn_flags<64, 32>(v) : fn_flags<64, 64>(v);
+    if (v.block == 128) return v.stack <= 24 ? fn_flags<128, 24>(v) : v.stack <= 32 ? fn_flags<128, 32>(v) : fn_flags<128, 64>(v);
+    return v.stack <= 24 ? fn_flags<256, 24>(v) : v.stack <= 32 ? fn_flags<256, 32>(v) : fn_flags<256, 64>(v);
 }
 }  // namespace
 
 uint32_t stack_variant(uint32_t needed)
 {
+    if (needed <= 24) return 24;   // 5 waves per SIMD
     if (needed <= 32) return 32;
     if (needed <= 64) return 64;
     return 0;
