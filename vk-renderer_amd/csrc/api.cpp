@@ -175,7 +175,7 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
     bi.sph = reinterpret_cast<const float*>(desc->spheres);
     bi.n_spheres = desc->n_spheres;
     bi.max_leaf = desc->max_leaf_size ? desc->max_leaf_size : default_leaf((uint32_t)em.size());
-    bi.max_depth = desc->max_depth ? desc->max_depth : 32u;
+    bi.max_depth = desc->max_depth ? desc->max_depth : kDefaultMaxDepth;
     bi.traversal_cost = desc->traversal_cost;
     // Host build: bo is the whole tree.  Device build: bo holds the sphere
     // records only; nodes and triangle records are already on the device.

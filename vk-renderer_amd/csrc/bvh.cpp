@@ -153,9 +153,18 @@ struct Builder {
         if (ext[1] > ext[axis]) axis = 1;
         if (ext[2] > ext[axis]) axis = 2;
 
-        // Near the depth cap use balanced median splits so every leaf fits.
-        const double need = std::ceil(std::log2((double)n)) + 1.0;
-        const bool forced_median = (double)dep + need >= (double)max_depth;
+        // Depth cap: median splits below a range of k primitives reach
+        // leaves of max_leaf after levels(k) more levels (one more for a range
+        // that mixes kinds).  SAH splits are taken only when both parts can
+        // still finish within the cap that way; a range with no such split,
+        // or no room left, is split at the median.
+        auto levels = [this](size_t k) -> uint32_t {
+            uint32_t l = 0;
+            while (((size_t)max_leaf << l) < k) ++l;
+            return l;
+        };
+        const uint32_t mixed = is_pure ? 0u : 1u;
+        const bool forced_median = dep + levels(n) + mixed >= max_depth;
 
         if (!forced_median && ext[axis] > 0.0f) {
             float best_cost = std::numeric_limits<float>::infinity();
@@ -190,6 +199,7 @@ struct Builder {
                     if (bc[b]) lacc.grow(bb[b].lo, bb[b].hi);
                     lcnt += bc[b];
                     if (lcnt == 0 || rn[b + 1] == 0) continue;
+                    if (dep + 1u + levels(lcnt) > max_depth || dep + 1u + levels(rn[b + 1]) > max_depth) continue;
                     const float cost = ctrav + (lacc.area() * (float)lcnt + ra[b + 1] * (float)rn[b + 1]) / parea;
                     if (cost < best_cost) {
                         best_cost = cost;
@@ -362,7 +372,7 @@ BuildOutput build_bvh(const BuildInput& in)
 
     Builder b;
     b.max_leaf = std::max(1u, std::min(in.max_leaf ? in.max_leaf : kDefaultLeaf, kMaxLeaf));
-    b.max_depth = std::max(8u, std::min(in.max_depth ? in.max_depth : 32u, 64u));
+    b.max_depth = std::max(8u, std::min(in.max_depth ? in.max_depth : kDefaultMaxDepth, 64u));
     // A node visit (four 128-bit loads, two slab tests, a stack step) costs
     // about three triangle tests on gfx950 (measured on the 100k soup).
     b.ctrav = in.traversal_cost > 0.0f ? in.traversal_cost : 3.0f;
