@@ -159,7 +159,7 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
     if ((desc->flags & VKR_SCENE_TRI_MATERIAL_INDEX) && (!desc->materials || desc->n_materials == 0))
         return fail(VKR_INVALID_ARGUMENT, "per-triangle materials requested without a material table");
     if (desc->max_leaf_size > kMaxLeaf) return fail(VKR_INVALID_ARGUMENT, "max_leaf_size must be <= 8");
-    if (desc->max_depth > 64) return fail(VKR_INVALID_ARGUMENT, "max_depth must be <= 64");
+    if (desc->max_depth > 63) return fail(VKR_INVALID_ARGUMENT, "max_depth must be <= 63");
     VKR_HIP(hipSetDevice(ctx->device));
 
     auto sc = std::make_unique<Scene>();
@@ -184,7 +184,7 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
     bool has_bvh = false;
     if ((desc->flags & VKR_SCENE_DEVICE_BVH) && desc->n_tris >= 2u) {
         hipError_t e = hipSuccess;
-        const uint32_t cap = desc->max_depth ? desc->max_depth : 64u;
+        const uint32_t cap = desc->max_depth ? desc->max_depth : 63u;
         if (device_build(ctx, sc.get(), desc->n_tris, bi, cap, bo, n_nodes, n_leaves, depth, &e)) {
             builder = VKR_BUILDER_DEVICE_LBVH;
             has_bvh = true;
@@ -206,8 +206,8 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
         VKR_HIP(sc->tris.upload(bo.tris.data(), bo.tris.size() * sizeof(TriRecord)));
     }
 
-    const uint32_t stack = has_bvh ? stack_variant(depth) : 32u;
-    if (stack == 0) return fail(VKR_UNSUPPORTED, "BVH deeper than 64 levels");
+    const uint32_t stack = has_bvh ? stack_variant(depth + 1u) : 32u;   // + the end marker
+    if (stack == 0) return fail(VKR_UNSUPPORTED, "BVH deeper than 63 levels");
 
     // Parameters as uploaded: light scale 1, emission = the sphere's emissive.
     sc->host_theta.assign(7u * desc->n_spheres + 1u, 0.0f);
@@ -347,7 +347,7 @@ vkr_error vkr_scene_set_sphere_params(vkr_ctx* ctx, uint32_t scene, const float*
     return VKR_SUCCESS;
 }
 
-// Trees of depth <= 24 can take the 24-entry-stack kernel, which keeps 5
+// Trees of depth <= 23 can take the 24-entry-stack kernel, which keeps 5
 // waves per SIMD resident (4 with deeper stacks) but spills more in its
 // per-step code.  The extra wave pays off only when the image has well over
 // the pixels the 4-wave kernel keeps resident (1080p soup frame: 90 ms

@@ -372,7 +372,7 @@ BuildOutput build_bvh(const BuildInput& in)
 
     Builder b;
     b.max_leaf = std::max(1u, std::min(in.max_leaf ? in.max_leaf : kDefaultLeaf, kMaxLeaf));
-    b.max_depth = std::max(8u, std::min(in.max_depth ? in.max_depth : kDefaultMaxDepth, 64u));
+    b.max_depth = std::max(8u, std::min(in.max_depth ? in.max_depth : kDefaultMaxDepth, 63u));
     // A node visit (four 128-bit loads, two slab tests, a stack step) costs
     // about three triangle tests on gfx950 (measured on the 100k soup).
     b.ctrav = in.traversal_cost > 0.0f ? in.traversal_cost : 3.0f;

@@ -49,11 +49,10 @@ constexpr uint32_t kMaxLeaf = 8;
 // 10.3 ms with 1).
 constexpr uint32_t kDefaultLeaf = 2;
 inline uint32_t default_leaf(uint32_t n_emitters) { return n_emitters >= 2u ? 1u : kDefaultLeaf; }
-// Default depth cap: trees of at most 24 levels run the path kernel with a
-// 24-entry stack, which keeps 5 waves per SIMD resident instead of 4.  The
-// SAH split search respects the cap (1M-triangle soup: 24 levels instead of
-// 25, frame 118 ms against 130 ms; the 2M-triangle heightfield keeps its 23).
-constexpr uint32_t kDefaultMaxDepth = 24;
+// Default depth cap: trees of at most 23 levels run the path kernel with a
+// 24-entry stack (one entry holds the end marker), which keeps 5 waves per
+// SIMD resident instead of 4.  The SAH split search respects the cap.
+constexpr uint32_t kDefaultMaxDepth = 23;
 
 struct alignas(16) BvhNode {
     float a[4], b[4], c[4];
@@ -102,8 +101,8 @@ void offset_inner_refs(std::vector<BvhNode>& nodes, uint32_t off);
 
 // Builds a binned-SAH BVH.  Leaves hold one primitive kind only and at most
 // max_leaf primitives; no leaf is deeper than max_depth (splits fall back to
-// balanced median splits near the cap), so a traversal stack of max_depth
-// entries never overflows.
+// balanced median splits near the cap), so a traversal stack of max_depth + 1
+// entries (one holds the end marker) never overflows.
 BuildOutput build_bvh(const BuildInput& in);
 
 }  // namespace vkr

@@ -64,13 +64,18 @@ __device__ __forceinline__ void cswap(float& ta, uint32_t& ka, float& tb, uint32
 //   any == true : only "was anything accepted" (shadow rays, :473).
 struct Ray {
     f3 o, d;
-    float tmax, tmax_eps, cull, tmin;
+    float tmax, tmax_eps, cb, tmin;     // cb: the box test's far bound (cull distance plus slack)
     float ix, iy, iz, ox, oy, oz, pad;   // |ix| etc. come free as source modifiers
     float ct0, ct1, ct2, ct3;
     uint32_t ck0, ck1, ck2, ck3;
     bool any, ovf, occluded, nan_ray;
     uint32_t ref;   // >= 0 (as int): inner node; < -1: leaf; kDone: traversal over
-    int sp;         // stack depth
+    int sp;         // stack entries; entry 0 is always kDone, so a pop needs no bounds check
+
+    // Boxes entered beyond `cull` hold no hit the ray can still accept; the
+    // box test compares against it with a relative and absolute slack, kept
+    // here so that node steps do not recompute it (it changes only at leaves).
+    __device__ __forceinline__ void set_cull(float cull) { cb = cull + fabsf(cull) * 1e-5f + 1e-5f; }
 
     __device__ __forceinline__ void start(const DevScene& sc, f3 o_, f3 d_, float tmax_, float eps, bool any_)
     {
@@ -80,7 +85,7 @@ struct Ray {
         tmax = tmax_;
         any = any_;
         tmax_eps = tmax + eps;
-        cull = any ? tmax_eps : tmax_eps + 4.0f * eps;
+        set_cull(any ? tmax_eps : tmax_eps + 4.0f * eps);
         tmin = inf;
         ct0 = ct1 = ct2 = ct3 = inf;
         ck0 = ck1 = ck2 = ck3 = 0;
@@ -101,7 +106,7 @@ struct Ray {
         const f3 oc = sub(o, f3{sc.sph_c[0], sc.sph_c[1], sc.sph_c[2]});
         pad = fminf(2e-3f * (sqrtf(dot(oc, oc)) + sc.sph_r), 3e38f);
         ref = (sc.has_bvh && !nan_ray) ? 0u : kDone;
-        sp = 0;
+        sp = 1;   // above the kDone entry the traversal loops keep at the bottom
     }
 
     // Visits inner node `ref`: near-first descent with a branch-free step.
@@ -117,7 +122,6 @@ struct Ray {
             wc.nodes++;
             count_wave(wc.wave_nodes);
         }
-        const float cb = cull + fabsf(cull) * 1e-5f + 1e-5f;
         // Slab test on centre/half-extent boxes: t = (c - o) i -+ (e + w) |i|,
         // with the widening w only in the sphere subtree (m = 1 there).
         const float mp = __uint_as_float(D.z) * pad;
@@ -135,15 +139,18 @@ struct Ray {
         const float f1 = fminf(fminf(b0 + b1, b2 + b3), fminf(b4 + b5, cb));
         const bool h0 = n0 <= f0, h1 = n1 <= f1;
         // The far child is always written to the free slot above the top
-        // (sp < STACK at any inner node, since the tree depth bounds the
-        // stack), and only counted as pushed when both children are hit.
+        // (sp < STACK at any inner node: the stack holds the kDone entry and
+        // at most one entry per level above, and the tree is at most
+        // STACK - 1 levels deep), and only counted as pushed when both
+        // children are hit.  With neither hit the top is popped; the bottom
+        // kDone entry ends the traversal.
         const bool first0 = n0 <= n1;
         const uint32_t nearr = (h0 && (!h1 || first0)) ? D.x : D.y;
         stk[sp * BLOCK] = first0 ? D.y : D.x;
-        const uint32_t top = stk[(sp > 0 ? sp - 1 : 0) * BLOCK];
+        const uint32_t top = stk[(sp - 1) * BLOCK];
         const bool none = !h0 && !h1;
-        ref = none ? (sp > 0 ? top : kDone) : nearr;
-        sp = (h0 && h1) ? sp + 1 : (none ? (sp > 0 ? sp - 1 : 0) : sp);
+        ref = none ? top : nearr;
+        sp += (h0 && h1 ? 1 : 0) - (none ? 1 : 0);
     }
 
     // Tests the primitives of leaf `ref`, then pops the stack.
@@ -201,14 +208,14 @@ struct Ray {
                         else { ct3 = t; ck3 = key; }
                     }
                 }
-                cull = fminf(tmin, tmax_eps) + win;
+                set_cull(fminf(tmin, tmax_eps) + win);
             }
         }
         if (occluded) {
             ref = kDone;
         } else {
-            ref = sp > 0 ? stk[(sp - 1) * BLOCK] : kDone;
-            sp = sp > 0 ? sp - 1 : 0;
+            ref = stk[(sp - 1) * BLOCK];   // the bottom entry is kDone
+            --sp;
         }
     }
 
@@ -291,6 +298,7 @@ __device__ __forceinline__ TravOut traverse(const DevScene& sc, f3 o, f3 d, floa
 {
     Ray r;
     r.start(sc, o, d, tmax, eps, any);
+    stk[0] = kDone;
     if (!active) r.ref = kDone;
     if (r.ref != kDone) {   // (only traversing lanes in the loop: 11 fewer VGPRs in k_path)
         while (true) {
@@ -382,6 +390,7 @@ __device__ __forceinline__ void traverse_pool(const JobDesc* jobs, uint32_t tota
     const JobDesc* rj = jobs;
     Ray r;
     r.ref = kDone;
+    stk[0] = kDone;
     uint32_t cur = kNone;
     uint32_t next = 0;   // wave-uniform pool cursor
     while (true) {
