@@ -246,7 +246,7 @@ extern "C" {
 // Builds the GPU's BVH over a scene (the arrays are copied) and returns the
 // accelerator to put in oscene.accel, with vkro_accel_fn() in oscene.accel_fn.
 // Same builder settings as vkr_scene_upload with default options (leaf size
-// from the emitter count, depth cap 32, default SAH cost).
+// from the emitter count, the default depth cap, default SAH cost).
 void* vkro_accel_build(const oscene* s)
 {
     Accel* A = new Accel();
@@ -268,6 +268,29 @@ void* vkro_accel_build(const oscene* s)
 }
 
 void vkro_accel_free(void* a) { delete static_cast<Accel*>(a); }
+
+// Builds the tree vkr_scene_upload would build with this leaf size and depth
+// cap (0 = the defaults) and reports out = {deepest leaf, nodes, leaves}.
+void vkro_bvh_shape(const oscene* s, uint32_t max_leaf, uint32_t max_depth, uint32_t out[3])
+{
+    uint32_t n_em = 0;
+    for (uint32_t i = 0; i < s->n_spheres; ++i) {
+        const float* e = s->spheres[i].mat.emissive;
+        if (!(e[0] == 0.0f && e[1] == 0.0f && e[2] == 0.0f)) ++n_em;
+    }
+    vkr::BuildInput in;
+    in.tri_v = s->n_tris ? reinterpret_cast<const float*>(s->tris) : nullptr;
+    in.tri_mat = nullptr;
+    in.n_tris = s->n_tris;
+    in.sph = s->n_spheres ? reinterpret_cast<const float*>(s->spheres) : nullptr;
+    in.n_spheres = s->n_spheres;
+    in.max_leaf = max_leaf ? max_leaf : vkr::default_leaf(n_em);
+    in.max_depth = max_depth ? max_depth : vkr::kDefaultMaxDepth;
+    const vkr::BuildOutput b = vkr::build_bvh(in);
+    out[0] = b.depth;
+    out[1] = (uint32_t)b.nodes.size();
+    out[2] = b.leaves;
+}
 
 oaccel_fn vkro_accel_fn(void) { return &trace; }
 

@@ -126,6 +126,8 @@ def load_accel():
         lib.vkro_accel_count.argtypes = [C.c_void_p, C.c_int]
         lib.vkro_accel_counts.restype = None
         lib.vkro_accel_counts.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+        lib.vkro_bvh_shape.restype = None
+        lib.vkro_bvh_shape.argtypes = [C.POINTER(OScene), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]
         _accel_lib = lib
     return _accel_lib
 
