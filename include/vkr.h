@@ -173,7 +173,8 @@ typedef struct vkr_scene_desc {
     const vkr_material* materials;   uint32_t n_materials; /* host; per-triangle table */
     uint32_t flags;                  /* VKR_SCENE_* */
     uint32_t max_leaf_size;          /* BVH leaf size cap (0 = 1 with 2+ emitters, else 2; at most 8) */
-    uint32_t max_depth;              /* host BVH depth cap (0 = 23, the deepest tree the 5-wave kernel takes; at most 63) */
+    uint32_t max_depth;              /* host BVH depth cap (0 = 23, the deepest tree the 5-wave kernel takes; at most 63;
+                                        a cap below the least depth the primitives allow gives that depth) */
     float    traversal_cost;         /* SAH cost of a node visit relative to one primitive test (0 = 3) */
 } vkr_scene_desc;
 
