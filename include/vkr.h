@@ -192,7 +192,8 @@ vkr_error vkr_scene_destroy(vkr_ctx* ctx, uint32_t scene);
 
 typedef struct vkr_scene_info {
     uint32_t n_tris, n_spheres, n_planes, n_emitters;
-    uint32_t n_nodes, n_leaves, depth, stack_needed;
+    uint32_t n_nodes, n_leaves, depth, stack_needed;   /* stack_needed: the traversal-stack variant
+                                                            (24, 32 or 64 entries) that holds depth + 1 */
     uint64_t device_bytes;           /* nodes + primitive records + materials */
     double   build_ms;               /* BVH build time (wall clock, device builds included) */
     uint32_t builder;                /* VKR_BUILDER_* */
