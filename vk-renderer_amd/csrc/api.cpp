@@ -206,7 +206,9 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
         VKR_HIP(sc->tris.upload(bo.tris.data(), bo.tris.size() * sizeof(TriRecord)));
     }
 
-    const uint32_t stack = has_bvh ? stack_variant(depth + 1u) : 32u;   // + the end marker
+    // depth + 1 entries: the end marker at the bottom.  Without a tree nothing
+    // is traversed, and the smallest variant serves.
+    const uint32_t stack = has_bvh ? stack_variant(depth + 1u) : 24u;
     if (stack == 0) return fail(VKR_UNSUPPORTED, "BVH deeper than 63 levels");
 
     // Parameters as uploaded: light scale 1, emission = the sphere's emissive.

@@ -23,7 +23,7 @@ namespace {
 struct BatchPlan {
     std::vector<JobDesc> jobs;
     std::vector<uint32_t> start;  // n + 1 pool offsets
-    uint32_t stack = 32;
+    uint32_t stack = 24;          // the smallest stack variant; raised to the jobs' needs
     uint64_t total = 0;
     bool pool = false;            // some job wants the pooled kernel
 };
