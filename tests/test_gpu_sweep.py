@@ -43,7 +43,10 @@ def _material(rng):
     return vkr.material(a, tuple(float(x) for x in rng.uniform(1.0, 64.0) * a), 0.6, 0.0)
 
 
-def _scene(seed):
+def _scene(seed, size=None):
+    """size=(w, h) replaces the drawn image size (after every draw, so the
+    scene, camera and trace settings stay those of the seed); spp then stays
+    at most 2."""
     rng = np.random.default_rng(1000 + seed)
     n_sph = int(rng.integers(2, 40))
     mats = [_material(rng) for _ in range(n_sph)]
@@ -65,9 +68,13 @@ def _scene(seed):
     cam = vkr.camera_from_angles(tuple(float(x) for x in pos), float(rng.uniform(-20, 20)),
                                  float(rng.uniform(70, 110)))
     w, h = int(rng.integers(17, 200)), int(rng.integers(13, 150))
-    fr = scenes.frame(cam, np.float32(w) / np.float32(h), seed=float(rng.integers(0, 1000)))
+    fseed = float(rng.integers(0, 1000))
     spp, depth = int(rng.integers(1, 4)), int(rng.integers(1, 7))
     eps = float(rng.choice([1e-3, 1e-3, 5e-4, 2e-3]))
+    if size is not None:
+        w, h = size
+        spp = min(spp, 2)
+    fr = scenes.frame(cam, np.float32(w) / np.float32(h), seed=fseed)
     return scenes.Workload(f"sweep{seed}", sc, fr, w, h, spp, depth), eps
 
 
@@ -91,6 +98,33 @@ def test_random_scene_matches_oracle(ctx, torch_cuda, seed, device_bvh, variant)
     st = compare(g, r)
     print(f"[sweep {seed}] " + " ".join(f"{k}={v}" for k, v in st.items()), flush=True)
     assert st["window_overflow"] == 0, st
+    assert st["prim_id"] == 1.0 and st["hit_t_bits"] == 1.0 and st["ray_count"] == 1.0, st
+    assert st["radiance_1e-4"] == 1.0 and st["rgba8"] == 1.0, st
+    ctx.destroy_scene(sid)
+
+
+@pytest.mark.parametrize("variant", ["auto", "other"], ids=["kernel_auto", "kernel_other"])
+@pytest.mark.parametrize("seed", range(6))
+def test_random_scene_large_frame(ctx, torch_cuda, seed, variant):
+    """The sweep's scenes at 1280x720: an image of more than twice the pixels
+    the 4-wave kernel keeps resident runs the 5-wave kernel (24-entry stack),
+    which the small frames above do not reach.  4096 random pixels and the
+    corners must equal the oracle's linear scan."""
+    wl, eps = _scene(seed, size=(1280, 720))
+    sid = ctx.upload(wl.scene)
+    info = ctx.info(sid)
+    print(f"\n[sweep720 {seed}] {len(wl.scene.tris)} tris {len(wl.scene.spheres)} spheres "
+          f"spp {wl.spp} depth {wl.max_depth} eps {eps} bvh depth {info['depth']} stack {info['stack_needed']}", flush=True)
+    flags = 0
+    if variant == "other":
+        flags = vkr.TRACE_RAY_POOL if info["n_emitters"] < 2 else vkr.TRACE_NO_RAY_POOL
+    g = gpu_trace_eps(ctx, sid, wl, eps, flags)
+    W, H = wl.width, wl.height
+    pix = np.random.default_rng(300 + seed).choice(W * H, 4096, replace=False).astype(np.uint32)
+    pix = np.unique(np.concatenate([pix, np.array([0, W - 1, (H - 1) * W, W * H - 1], np.uint32)]))
+    r = oracle_ffi.trace(wl.scene, wl.frame, W, H, wl.spp, wl.max_depth, epsilon=eps, pixels=pix)
+    st = compare(g, r, pixels=pix)
+    print(f"[sweep720 {seed}] " + " ".join(f"{k}={v}" for k, v in st.items()), flush=True)
     assert st["prim_id"] == 1.0 and st["hit_t_bits"] == 1.0 and st["ray_count"] == 1.0, st
     assert st["radiance_1e-4"] == 1.0 and st["rgba8"] == 1.0, st
     ctx.destroy_scene(sid)
