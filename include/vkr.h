@@ -358,8 +358,11 @@ vkr_error vkr_autotune_batch(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n
  * d_radiance (device, W*H*4 floats, dLoss/d(mean radiance rgb); a ignored)
  * accumulates dLoss/dparams into d_params (device, 7 * n_spheres + 1 floats,
  * layout of vkr_scene_set_sphere_params; the caller zeroes it).  Paths are
- * recomputed from the RNG stream, not stored.  Branch decisions (emitter set,
- * Russian roulette, dielectric choice) are held fixed. */
+ * recomputed from the RNG stream, not stored: a first pass keeps 8 bytes per
+ * (pixel, sample) in a context buffer (grown on demand, which waits for the
+ * device) so that the second replays every sample in its own lane.  Branch
+ * decisions (emitter set, Russian roulette, dielectric choice) are held
+ * fixed. */
 vkr_error vkr_adjoint(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* frame_data,
                       const vkr_trace_params* params, const float* d_radiance,
                       float* d_params, void* stream);

@@ -7,12 +7,16 @@
 // emitter set, Russian roulette, the dielectric choice), i.e. the exact
 // derivative of the per-sample estimate of Assets/Tracer.comp:433-593.
 //
-// Path replay: each sample is traced twice from its RNG state and nothing of
-// the path is stored.
+// Path replay: each sample is traced twice from its RNG state; of the path
+// only 8 bytes per sample are stored between the two passes.
 //   pass 1 records which channels of clamp(acc) (Tracer.comp:441) saturate,
-//          3 bits per bounce; a saturated channel blocks every earlier term;
+//          3 bits per bounce; a saturated channel blocks every earlier term.
+//          A pixel's samples share one random stream, so this pass runs them
+//          in order, one lane per pixel, and keeps for each sample the stream
+//          position at its start and its saturation bits;
 //   pass 2 re-traces the same path (same random numbers, same shadow rays)
-//          and accumulates, with W_k = dLoss/dL masked by the later clamps:
+//          from that position, one lane per (pixel, sample), and accumulates,
+//          with W_k = dLoss/dL masked by the later clamps:
 //   * direct terms of q_k (normalize(emissive), the next-event term, the
 //     dielectric emissive) in a_s, r_s, e, L at each vertex k;
 //   * the throughput chain.  Because Russian roulette divides the mask by
@@ -40,7 +44,10 @@ __device__ __forceinline__ uint32_t argmax3(f3 v)
 }
 __device__ __forceinline__ float comp(f3 v, uint32_t c) { return c == 0 ? v.x : (c == 1 ? v.y : v.z); }
 
-template <int BLOCK, int STACK, bool LDSGRAD>
+// MODE 1: pass 1 over every sample of a pixel, recording A.samples;
+// MODE 2: pass 2 of one sample per work item (items: per 8x8 tile, sample by
+// sample, so a wave replays one sample of 64 neighbouring pixels).
+template <int BLOCK, int STACK, bool LDSGRAD, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_adjoint(const AdjointLaunch A)
 {
     extern __shared__ uint32_t s_dyn[];
@@ -60,7 +67,9 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(const AdjointLaunch A)
 
     bool has = false, exhausted = false;
     float uvx = 0.0f, uvy = 0.0f, salt = 0.0f, salt0 = 0.0f;
-    uint32_t smp = 0, depth = 0, phase = kPhTrace, pass = 1, zb = 0, nchain = 0, cstar = 0, hs = kMiss;
+    uint32_t smp = 0, depth = 0, phase = kPhTrace, pass = MODE, zb = 0, nchain = 0, cstar = 0, hs = kMiss;
+    uint32_t pix = 0;
+    uint32_t* const cursor = A.pool + (MODE == 2 ? 1 : 0);
     f3 ro{}, rd{}, acc{}, mask{}, P{}, N{}, dj{}, e{}, cnee{}, gp{};
     f3 gA{}, gE{};
     float gR = 0.0f, gL = 0.0f;
@@ -72,18 +81,30 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(const AdjointLaunch A)
     WorkCnt wc{};
 
     auto claim = [&](uint32_t g) __attribute__((always_inline)) {
-        const uint32_t tile = g >> 6, within = g & 63u;
+        uint32_t tile = g >> 6, s = 0;
+        if (MODE == 2) {
+            tile = g / (64u * j.spp);
+            s = (g >> 6) % j.spp;
+        }
+        const uint32_t within = g & 63u;
         if (tile >= j.n_tiles) return;
         const uint32_t x = (tile % j.tiles_x) * 8u + (within & 7u);
         const uint32_t y = (tile / j.tiles_x) * 8u + (within >> 3);
         if (x >= j.W || y >= j.H) return;
         has = true;
+        pix = y * j.W + x;
         uvx = (float)x / (float)j.W;
         uvy = (float)y / (float)j.H;
-        salt = uvy / uvx + j.seed;
-        smp = 0;
-        pass = 1;
-        const float4 g4 = reinterpret_cast<const float4*>(A.d_radiance)[(size_t)y * j.W + x];
+        smp = s;
+        pass = MODE;
+        if (MODE == 2) {
+            const uint2 rec = A.samples[(size_t)pix * j.spp + s];
+            salt0 = __uint_as_float(rec.x);
+            zb = rec.y;
+        } else {
+            salt = uvy / uvx + j.seed;
+        }
+        const float4 g4 = reinterpret_cast<const float4*>(A.d_radiance)[pix];
         const float inv = 1.0f / (float)j.spp;      // d(mean)/d(sample) (:585)
         gp = f3{g4.x * inv, g4.y * inv, g4.z * inv};
         new_path = true;
@@ -120,7 +141,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(const AdjointLaunch A)
             if (nneed > 0 && (nneed >= 32u || busy == 0)) {
                 const int leader = __ffsll((long long)need) - 1;
                 uint32_t base = 0;
-                if (__lane_id() == leader) base = atomicAdd(A.pool, nneed);
+                if (__lane_id() == leader) base = atomicAdd(cursor, nneed);
                 base = __shfl(base, leader, 64);
                 if (base + nneed >= A.total) exhausted = true;
                 if (!has) {
@@ -384,14 +405,14 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(const AdjointLaunch A)
             }
         }
         if (path_end) {
-            if (pass == 1) {
-                pass = 2;
-                new_path = true;
-            } else {
-                pass = 1;
+            if (MODE == 1) {
+                // The next sample starts where this one's draws ended.
+                A.samples[(size_t)pix * j.spp + smp] = make_uint2(__float_as_uint(salt0), zb);
                 ++smp;
                 if (smp < j.spp) new_path = true;
                 else has = false;
+            } else {
+                has = false;
             }
         }
     }
@@ -405,11 +426,32 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(const AdjointLaunch A)
 }
 
 namespace {
-template <int BLOCK, int STACK, bool LDSGRAD>
-hipError_t go(const AdjointLaunch& a, uint32_t grid, size_t lds, hipStream_t s)
+template <int BLOCK, int STACK, bool LDSGRAD, int MODE>
+hipError_t go(const AdjointLaunch& a, size_t lds, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_adjoint<BLOCK, STACK, LDSGRAD>), dim3(grid), dim3(BLOCK), lds, s, a);
+    const void* fn = reinterpret_cast<const void*>(&k_adjoint<BLOCK, STACK, LDSGRAD, MODE>);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BLOCK, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 256;
+    const uint32_t need = (a.total + BLOCK - 1u) / BLOCK;
+    const uint32_t grid = std::max(1u, std::min((uint32_t)(per_cu * cus), need));
+    hipLaunchKernelGGL((k_adjoint<BLOCK, STACK, LDSGRAD, MODE>), dim3(grid), dim3(BLOCK), lds, s, a);
     return hipGetLastError();
+}
+
+template <int BLOCK, int STACK>
+hipError_t both_passes(AdjointLaunch a, bool ldsgrad, size_t base, size_t grad_bytes, hipStream_t s)
+{
+    // Pass 1: one lane per pixel (pool indices as in the path kernel).
+    const uint32_t pixels = a.total;
+    hipError_t e = go<BLOCK, STACK, false, 1>(a, base, s);
+    if (e != hipSuccess) return e;
+    // Pass 2: one lane per (pixel, sample).
+    a.total = pixels * a.job.spp;
+    return ldsgrad ? go<BLOCK, STACK, true, 2>(a, base + grad_bytes, s) : go<BLOCK, STACK, false, 2>(a, base, s);
 }
 }  // namespace
 
@@ -419,19 +461,9 @@ hipError_t launch_adjoint(const AdjointLaunch& a, uint32_t stack, hipStream_t s)
     const uint32_t np = 7u * a.job.sc.n_spheres + 1u;
     const size_t base = (size_t)((stack <= 32 ? 32 : 64) + kMaxAdjDepth) * kBlock * 4u;
     const bool lds = base + (size_t)np * 4u <= 60u * 1024u;
-    const size_t bytes = base + (lds ? (size_t)np * 4u : 0u);
-    const void* fn = stack <= 32 ? (lds ? (const void*)&k_adjoint<kBlock, 32, true> : (const void*)&k_adjoint<kBlock, 32, false>)
-                                 : (lds ? (const void*)&k_adjoint<kBlock, 64, true> : (const void*)&k_adjoint<kBlock, 64, false>);
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, bytes) != hipSuccess || per_cu < 1) per_cu = 1;
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
-        cus = 256;
-    const uint32_t need = (a.total + kBlock - 1u) / kBlock;
-    const uint32_t grid = std::max(1u, std::min((uint32_t)(per_cu * cus), need));
-    if (stack <= 32) return lds ? go<kBlock, 32, true>(a, grid, bytes, s) : go<kBlock, 32, false>(a, grid, bytes, s);
-    return lds ? go<kBlock, 64, true>(a, grid, bytes, s) : go<kBlock, 64, false>(a, grid, bytes, s);
+    const size_t grad_bytes = (size_t)np * 4u;
+    if (stack <= 32) return both_passes<kBlock, 32>(a, lds, base, grad_bytes, s);
+    return both_passes<kBlock, 64>(a, lds, base, grad_bytes, s);
 }
 
 }  // namespace vkr

@@ -639,6 +639,14 @@ vkr_error vkr_adjoint(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd, co
     a.counters = ctx->counters.as<DevCounters>();
     a.pool = ctx->pool.as<uint32_t>();
     a.total = a.job.n_tiles * 64u;
+    // 8 bytes per (pixel, sample) between the two passes; growing the buffer
+    // waits for the device, as for the ray records.
+    const size_t rec_bytes = (size_t)a.job.W * a.job.H * a.job.spp * sizeof(uint2);
+    if (ctx->adj_samples.bytes < rec_bytes) {
+        VKR_HIP(hipDeviceSynchronize());
+        VKR_HIP(ctx->adj_samples.alloc(rec_bytes));
+    }
+    a.samples = ctx->adj_samples.as<uint2>();
     VKR_HIP(launch_zero_counters(a.counters, a.pool, st));
     VKR_HIP(hipEventRecord(ctx->ev0, st));
     VKR_HIP(launch_adjoint(a, s->dev.stack_needed, st));

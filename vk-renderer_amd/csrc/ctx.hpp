@@ -118,6 +118,7 @@ struct vkr_ctx {
     vkr::DevBuf counters, pool, jobs, job_start;
     vkr::DevBuf lanes;                 // per-lane ray records of direct launches (grown on demand)
     vkr::DevBuf build_scratch;         // device BVH builds (grown on demand, reused across uploads)
+    vkr::DevBuf adj_samples;           // the adjoint's per-sample records (grown on demand)
     void* host_jobs = nullptr;         // pinned staging for batched job tables
     size_t host_jobs_bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_copy = nullptr;

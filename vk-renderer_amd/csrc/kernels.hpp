@@ -118,8 +118,10 @@ struct AdjointLaunch {
     float* d_params;              // 7 * n_spheres + 1
     const float* params;          // 7 * n_spheres + 1 (current values)
     DevCounters* counters;
-    uint32_t* pool;               // persistent pool cursor
-    uint32_t total;               // pool indices (64 per 8x8 tile)
+    uint32_t* pool;               // persistent pool cursors: [0] pass 1, [1] pass 2
+    uint32_t total;               // pool indices (64 per 8x8 tile; pass 2: per tile and sample)
+    uint2* samples;               // W*H*spp: (bits of the random-stream position at the sample's
+                                  // start, the pass-1 clamp-saturation bits), pixel-major
 };
 
 // One instantiation of the path kernel.  pool: a wave traces up to

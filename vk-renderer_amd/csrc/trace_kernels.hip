@@ -926,7 +926,10 @@ __global__ void k_zero(DevCounters* c, uint32_t* pool)
     if (threadIdx.x == 0) {
         c->rays = c->shadow = c->traced = c->nodes = c->tris = c->spheres = c->planes = c->overflow = 0ull;
         c->wave_nodes = c->wave_leaves = c->wave_steps = 0ull;
-        if (pool) *pool = 0u;
+        if (pool) {
+            pool[0] = 0u;
+            pool[1] = 0u;   // the adjoint's second pass
+        }
     }
 }
 
