@@ -102,6 +102,41 @@ def test_adjoint_matches_finite_differences(ctx, torch_cuda):
     assert np.all(g[[7 * s + 3 for s in range(n_sph - 1)]] == 0.0)
 
 
+def test_adjoint_odd_size_matches_finite_differences(ctx, torch_cuda):
+    """45x29 (partial 8x8 tiles) at 3 spp: the second pass takes one work item
+    per (pixel, sample), a tile's 64 pixels sample by sample, and must still
+    give every pixel's samples their own stream positions."""
+    torch = torch_cuda
+    w, h, spp = 45, 29, 3
+    wl = scenes.adjoint_scene(width=w, height=h, spp=spp)
+    sid = ctx.upload(wl.scene)
+    theta = scenes.sphere_params(wl.scene.spheres, light_scale=1.0)
+    dec = torch.zeros(w * h * spp, dtype=torch.int16, device="cuda")
+    base = _forward(ctx, torch, sid, wl, theta, vkr.TRACE_RECORD_DECISIONS, dec)
+    weights = np.random.default_rng(3).uniform(-1.0, 1.0, base.shape)
+    g4 = torch.zeros((w * h, 4), dtype=torch.float32, device="cuda")
+    g4[:, :3] = torch.from_numpy(weights.astype(np.float32)).cuda()
+    d = torch.zeros(theta.size, dtype=torch.float32, device="cuda")
+    ctx.set_sphere_params(sid, theta)
+    ctx.adjoint(sid, wl.frame, wl.params(), g4, d)
+    torch.cuda.synchronize()
+    g = d.double().cpu().numpy()
+    n_sph = len(wl.scene.spheres)
+    idx = [7 * s + k for s in range(n_sph - 1) for k in (0, 1, 2, 6)]
+    idx += [7 * (n_sph - 1) + k for k in (3, 4, 5)] + [7 * n_sph]
+    fd = np.array([_fd(ctx, torch, sid, wl, theta, dec, i, weights) for i in idx])
+    ga = g[idx]
+    scale = np.abs(fd).max()
+    big = np.abs(fd) > 1e-3 * scale
+    agg = np.linalg.norm(ga[big] - fd[big]) / np.linalg.norm(fd[big])
+    rel = np.abs(ga - fd) / np.maximum(np.abs(fd), 1e-3 * scale)
+    print(f"\n[adjoint {w}x{h}x{spp}] {big.sum()} parameters, aggregate relative error {agg:.3g}, "
+          f"{(rel[big] <= 1e-3).mean() * 100:.1f}% within 1e-3")
+    assert big.sum() >= 20
+    assert agg <= 1e-3
+    ctx.destroy_scene(sid)
+
+
 def test_adjoint_cost(ctx, torch_cuda):
     torch = torch_cuda
     import time
