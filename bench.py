@@ -23,7 +23,9 @@ Prints ONE JSON line.  Besides the headline it reports:
                 ONE batched persistent launch replayed from a captured
                 hipGraph, at the default and at the autotuned shape;
   heightfield   the 2.09M-triangle heightfield, 1080p, 8 spp, 8 bounces;
-  adjoint       the 256^2 / 16 spp adjoint workload, forward vs adjoint.
+  adjoint       the 256^2 / 16 spp adjoint workload: forward, adjoint on its
+                own, and the forward + adjoint pair of a differentiable render
+                step (the forward keeps the adjoint's first pass).
 """
 from __future__ import annotations
 
@@ -101,9 +103,16 @@ def host_cpu():
 
 def bench_adjoint(ctx, torch, repeats=5):
     """Adjoint workload (256x256, 16 spp, 4 bounces, 32 diffuse/metal spheres
-    + 1 light): forward radiance pass vs the path-replay adjoint pass with a
-    dense dLoss/dradiance, each timed over `repeats` launches."""
+    + 1 light) with a dense dLoss/dradiance, each figure over `repeats` calls:
+    the forward radiance pass; the adjoint on its own (both passes: calls
+    alternate between two frame seeds, so none finds its first pass kept);
+    the forward that keeps the adjoint's first pass (VKR_TRACE_RECORD_ADJOINT)
+    and the adjoint that then runs only its second pass; and the pair a
+    differentiable render runs per step (recording forward, then adjoint)."""
+    import numpy as np
+
     import scenes
+    import vkr
 
     wl = scenes.adjoint_scene()
     sid = ctx.upload(wl.scene)
@@ -112,9 +121,25 @@ def bench_adjoint(ctx, torch, repeats=5):
     g4 = torch.ones((n, 4), dtype=torch.float32, device="cuda")
     d = torch.zeros(7 * len(wl.scene.spheres) + 1, dtype=torch.float32, device="cuda")
     p = wl.params()
+    pr = wl.params(flags=vkr.TRACE_RECORD_ADJOINT)
+    frames = [wl.frame.copy(), wl.frame.copy()]
+    frames[1]["seed"] = frames[0]["seed"] + np.float32(1.0)
+    it = {"k": 0}
+
+    def alternating():
+        it["k"] ^= 1
+        ctx.adjoint(sid, frames[it["k"]], p, g4, d)
+
+    def pair():
+        ctx.trace(sid, wl.frame, pr, radiance=rad)
+        ctx.adjoint(sid, wl.frame, p, g4, d)
+
     res = {}
     for name, fn in (("forward", lambda: ctx.trace(sid, wl.frame, p, radiance=rad)),
-                     ("adjoint", lambda: ctx.adjoint(sid, wl.frame, p, g4, d))):
+                     ("adjoint", alternating),
+                     ("forward_recording", lambda: ctx.trace(sid, wl.frame, pr, radiance=rad)),
+                     ("adjoint_recorded", lambda: ctx.adjoint(sid, wl.frame, p, g4, d)),
+                     ("forward_backward", pair)):
         fn()
         torch.cuda.synchronize()
         ts = time.perf_counter()
@@ -126,9 +151,12 @@ def bench_adjoint(ctx, torch, repeats=5):
     torch.cuda.synchronize()
     rays = ctx.counters()["rays"]
     ctx.destroy_scene(sid)
-    return {"workload": "adjoint_256x256_16spp_4bounce_33spheres", "forward_ms": round(res["forward"] * 1e3, 3),
-            "adjoint_ms": round(res["adjoint"] * 1e3, 3),
-            "adjoint_over_forward": round(res["adjoint"] / res["forward"], 3),
+    ms = {k: round(v * 1e3, 3) for k, v in res.items()}
+    return {"workload": "adjoint_256x256_16spp_4bounce_33spheres", "forward_ms": ms["forward"],
+            "adjoint_ms": ms["adjoint"], "adjoint_over_forward": round(res["adjoint"] / res["forward"], 3),
+            "forward_recording_ms": ms["forward_recording"], "adjoint_recorded_ms": ms["adjoint_recorded"],
+            "forward_backward_ms": ms["forward_backward"],
+            "forward_backward_over_forward": round(res["forward_backward"] / res["forward"], 3),
             "forward_mrays": round(rays / res["forward"] / 1e6, 2),
             "adjoint_mrays": round(rays / res["adjoint"] / 1e6, 2)}
 

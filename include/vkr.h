@@ -212,7 +212,8 @@ vkr_error vkr_scene_get_info(vkr_ctx* ctx, uint32_t scene, vkr_scene_info* out);
  * metalness stay) and recomputes the emitter list in place (its device
  * buffer holds every sphere, so it never moves); captured graphs that trace
  * the scene see the new list on their next launch.  Synchronises the device.
- * Host pointer.
+ * Host pointer.  Values that differ from the current ones in any bit make the
+ * adjoint's kept first-pass records stale (see vkr_adjoint).
  * Build extension: the reference has no parameters (Tracer.comp:186-202). */
 vkr_error vkr_scene_set_sphere_params(vkr_ctx* ctx, uint32_t scene, const float* params, uint32_t n);
 
@@ -258,7 +259,16 @@ enum {
      * lane and step from a shared pool, else one ray per lane and step.
      * These force one or the other; the results are identical either way. */
     VKR_TRACE_RAY_POOL         = 1u << 2,
-    VKR_TRACE_NO_RAY_POOL      = 1u << 3
+    VKR_TRACE_NO_RAY_POOL      = 1u << 3,
+    /* vkr_trace_frame only (max_depth <= 10, no decision replay): also keeps,
+     * in the context, what the first pass of vkr_adjoint computes for this
+     * frame (each sample's random-stream position at its start and which
+     * channels of its clamp saturate, 8 bytes per sample).  The next
+     * vkr_adjoint with the same scene, sphere parameters, frame data and
+     * trace parameters then runs only its second pass.  The image is
+     * unchanged.  Refused by vkr_trace_batch, vkr_batch_graph_create and
+     * vkr_trace_frame_static. */
+    VKR_TRACE_RECORD_ADJOINT   = 1u << 4
 };
 
 typedef struct vkr_trace_params {
@@ -360,9 +370,14 @@ vkr_error vkr_autotune_batch(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n
  * layout of vkr_scene_set_sphere_params; the caller zeroes it).  Paths are
  * recomputed from the RNG stream, not stored: a first pass keeps 8 bytes per
  * (pixel, sample) in a context buffer (grown on demand, which waits for the
- * device) so that the second replays every sample in its own lane.  Branch
- * decisions (emitter set, Russian roulette, dielectric choice) are held
- * fixed. */
+ * device) so that the second replays every sample in its own lane.  The
+ * first pass is skipped when the buffer already holds its records for the
+ * same scene, sphere parameters, frame data and trace parameters (width,
+ * height, spp, max_depth, epsilon): written by a vkr_trace_frame with
+ * VKR_TRACE_RECORD_ADJOINT or by the previous vkr_adjoint.  Those records
+ * are read in stream order: run the recording call on the same stream or
+ * order the streams.  Branch decisions (emitter set, Russian roulette,
+ * dielectric choice) are held fixed. */
 vkr_error vkr_adjoint(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* frame_data,
                       const vkr_trace_params* params, const float* d_radiance,
                       float* d_params, void* stream);

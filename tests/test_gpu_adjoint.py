@@ -147,11 +147,16 @@ def test_adjoint_cost(ctx, torch_cuda):
     g4 = torch.ones((n, 4), dtype=torch.float32, device="cuda")
     d = torch.zeros(7 * len(wl.scene.spheres) + 1, dtype=torch.float32, device="cuda")
 
+    frames = [wl.frame, _other(wl.frame)]
+    k = [0]
+
     def fwd():
         ctx.trace(sid, wl.frame, wl.params(), radiance=rad)
 
     def adj():
-        ctx.adjoint(sid, wl.frame, wl.params(), g4, d)
+        # Alternating frame seeds: every call runs both passes.
+        k[0] ^= 1
+        ctx.adjoint(sid, frames[k[0]], wl.params(), g4, d)
 
     res = {}
     for name, fn in (("forward", fwd), ("adjoint", adj)):
@@ -315,3 +320,166 @@ def test_adjoint_matches_oracle_finite_differences(ctx, torch_cuda):
         r2 = abs(ga[k] - f_small) / abs(f_small)
         print(f"  param {idx[k]}: rel {rel[k]:.3g} at h=1e-3 -> {r2:.3g} at h=6.25e-5")
         assert r2 < rel[k] and r2 <= 5e-3
+
+
+def _adjoint(ctx, torch, sid, frame, params, g4, n_par):
+    """Gradient and the adjoint's ray count (both passes, or the second only
+    when the first pass's records were kept)."""
+    d = torch.zeros(n_par, dtype=torch.float32, device="cuda")
+    ctx.adjoint(sid, frame, params, g4, d)
+    torch.cuda.synchronize()
+    return d.double().cpu().numpy(), ctx.counters()["rays"]
+
+
+def _other(frame):
+    f = frame.copy()
+    f["seed"] = frame["seed"] + np.float32(1.0)
+    return f
+
+
+def _images(ctx, torch, sid, wl, flags):
+    n = wl.width * wl.height
+    rgba = torch.zeros((n, 4), dtype=torch.uint8, device="cuda")
+    rad = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+    ctx.trace(sid, wl.frame, wl.params(flags=flags), rgba8=rgba, radiance=rad)
+    torch.cuda.synchronize()
+    return rgba.cpu().numpy(), rad.cpu().numpy()
+
+
+@pytest.mark.parametrize("case", ["adjoint_bright", "soup_pooled", "many_emitters"])
+def test_recording_forward_keeps_image_and_gradient(ctx, torch_cuda, case):
+    """VKR_TRACE_RECORD_ADJOINT: the forward image is bit-identical with and
+    without it, and the adjoint that follows runs only its second pass (about
+    half the rays) on records equal to those its own first pass writes: the
+    gradient matches the adjoint run on its own up to the order of the float
+    additions that reduce it.  The adjoint scene runs with light scale 4, so
+    that many samples saturate the per-bounce clamp (whose bits the records
+    carry); the soup (several emitters) and the many-emitter scene run the
+    pooled kernel."""
+    torch = torch_cuda
+    if case == "adjoint_bright":
+        wl = scenes.adjoint_scene(width=128, height=96, spp=8)
+        scale = 4.0
+    elif case == "soup_pooled":
+        wl = scenes.soup100k(n_tris=20_000, width=160, height=120, spp=2, max_depth=4)
+        scale = 1.0
+    else:
+        wl = scenes.many_emitters()
+        scale = 1.0
+    sid = ctx.upload(wl.scene)
+    theta = scenes.sphere_params(wl.scene.spheres, light_scale=scale)
+    ctx.set_sphere_params(sid, theta)
+    n = wl.width * wl.height
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    g4 = (torch.rand((n, 4), device="cuda", generator=gen) - 0.5).contiguous()
+    p = wl.params()
+    # On its own: the previous call was for another frame, so both passes run.
+    _adjoint(ctx, torch, sid, _other(wl.frame), p, g4, theta.size)
+    g_own, rays_own = _adjoint(ctx, torch, sid, wl.frame, p, g4, theta.size)
+    # After a recording forward: the images do not change, pass 1 is skipped.
+    img0 = _images(ctx, torch, sid, wl, 0)
+    img1 = _images(ctx, torch, sid, wl, vkr.TRACE_RECORD_ADJOINT)
+    assert np.array_equal(img0[0], img1[0]) and np.array_equal(img0[1], img1[1])
+    g_rec, rays_rec = _adjoint(ctx, torch, sid, wl.frame, p, g4, theta.size)
+    # A second adjoint with the same inputs reuses its own records too.
+    g_again, rays_again = _adjoint(ctx, torch, sid, wl.frame, p, g4, theta.size)
+    agg = np.linalg.norm(g_rec - g_own) / np.linalg.norm(g_own)
+    agg2 = np.linalg.norm(g_again - g_own) / np.linalg.norm(g_own)
+    print(f"\n[recorded adjoint {case}] rays both passes {rays_own} second only {rays_rec}; "
+          f"relative difference {agg:.3g} / {agg2:.3g}")
+    assert np.linalg.norm(g_own) > 0
+    assert agg <= 1e-5 and agg2 <= 1e-5
+    assert rays_rec == rays_again and 0 < rays_rec < rays_own
+    ctx.destroy_scene(sid)
+
+
+def test_recording_goes_stale(ctx, torch_cuda):
+    """The kept first pass serves only the same scene version, sphere
+    parameters, frame data and trace parameters: any change runs both passes
+    again (and gives that input's gradient); setting the same parameter
+    values again keeps it."""
+    torch = torch_cuda
+    wl = scenes.adjoint_scene(width=96, height=64, spp=4)
+    sid = ctx.upload(wl.scene)
+    theta = scenes.sphere_params(wl.scene.spheres, light_scale=2.0)
+    theta2 = theta.copy()
+    theta2[7 * len(wl.scene.spheres)] = np.float32(3.0)
+    n = wl.width * wl.height
+    g4 = torch.ones((n, 4), dtype=torch.float32, device="cuda")
+    p = wl.params()
+
+    def with_record(params):
+        q = vkr.TraceParams.from_buffer_copy(params)
+        q.flags |= vkr.TRACE_RECORD_ADJOINT
+        return q
+
+    def record(th, frame=wl.frame, params=p):
+        ctx.set_sphere_params(sid, th)
+        ctx.trace(sid, frame, with_record(params),
+                  radiance=torch.zeros((n, 4), dtype=torch.float32, device="cuda"))
+
+    def own(th, frame=wl.frame, params=p):
+        ctx.set_sphere_params(sid, th)
+        _adjoint(ctx, torch, sid, _other(frame), params, g4, theta.size)
+        return _adjoint(ctx, torch, sid, frame, params, g4, theta.size)
+
+    def close(g, ref):
+        return np.linalg.norm(g - ref) <= 1e-5 * np.linalg.norm(ref)
+
+    g1, both1 = own(theta)
+    g2, both2 = own(theta2)
+    # Recorded at theta: the same values set again keep the records.
+    record(theta)
+    ctx.set_sphere_params(sid, theta.copy())
+    g, r = _adjoint(ctx, torch, sid, wl.frame, p, g4, theta.size)
+    assert r < both1 and close(g, g1)
+    # New parameter values: both passes, theta2's gradient.
+    record(theta)
+    ctx.set_sphere_params(sid, theta2)
+    g, r = _adjoint(ctx, torch, sid, wl.frame, p, g4, theta.size)
+    assert r == both2 and close(g, g2)
+    assert np.linalg.norm(g1 - g2) > 1e-3 * np.linalg.norm(g2)
+    # Another frame seed, sample count or depth: both passes.
+    for frame, params in ((_other(wl.frame), p),
+                          (wl.frame, vkr.Context.params(wl.width, wl.height, 3, wl.max_depth)),
+                          (wl.frame, vkr.Context.params(wl.width, wl.height, wl.spp, 3))):
+        g_ref, both = own(theta, frame, params)
+        record(theta)
+        g, r = _adjoint(ctx, torch, sid, frame, params, g4, theta.size)
+        assert r == both and close(g, g_ref)
+    # A scene uploaded into a destroyed scene's id is another scene, even
+    # with the same contents: recorded at the parameters an upload starts
+    # with, the new scene's adjoint still runs both passes.
+    theta0 = scenes.sphere_params(wl.scene.spheres, light_scale=1.0)
+    g0, both0 = own(theta0)
+    record(theta0)
+    ctx.destroy_scene(sid)
+    sid2 = ctx.upload(wl.scene)
+    assert sid2 == sid
+    g, r = _adjoint(ctx, torch, sid2, wl.frame, p, g4, theta.size)
+    assert r == both0 and close(g, g0)
+    ctx.destroy_scene(sid2)
+
+
+def test_recording_flag_refusals(ctx, torch_cuda):
+    torch = torch_cuda
+    wl = scenes.adjoint_scene(width=64, height=64, spp=2)
+    sid = ctx.upload(wl.scene)
+    rad = torch.zeros((64 * 64, 4), dtype=torch.float32, device="cuda")
+    dec = torch.zeros(64 * 64 * 2, dtype=torch.int16, device="cuda")
+    R = vkr.TRACE_RECORD_ADJOINT
+    bad = [
+        lambda: ctx.trace(sid, wl.frame, wl.params(flags=R, kernel=vkr.KERNEL_WHITTED), radiance=rad),
+        lambda: ctx.trace(sid, wl.frame, vkr.Context.params(64, 64, 2, 11, flags=R), radiance=rad),
+        lambda: ctx.trace(sid, wl.frame, wl.params(flags=R | vkr.TRACE_RECORD_DECISIONS), radiance=rad,
+                          decisions=dec),
+        lambda: ctx.trace_static(sid, wl.frame, wl.params(flags=R), radiance=rad),
+        lambda: ctx.trace_batch([ctx.job(sid, wl.frame, wl.params(flags=R), radiance=rad)]),
+        lambda: ctx.batch_graph([ctx.job(sid, wl.frame, wl.params(flags=R), radiance=rad)]),
+    ]
+    for fn in bad:
+        with pytest.raises(vkr.VkrError):
+            fn()
+    ctx.trace(sid, wl.frame, wl.params(flags=R), radiance=rad)
+    torch.cuda.synchronize()
+    ctx.destroy_scene(sid)

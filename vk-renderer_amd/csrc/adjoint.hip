@@ -443,27 +443,30 @@ hipError_t go(const AdjointLaunch& a, size_t lds, hipStream_t s)
 }
 
 template <int BLOCK, int STACK>
-hipError_t both_passes(AdjointLaunch a, bool ldsgrad, size_t base, size_t grad_bytes, hipStream_t s)
+hipError_t both_passes(AdjointLaunch a, bool pass1, bool ldsgrad, size_t base, size_t grad_bytes, hipStream_t s)
 {
     // Pass 1: one lane per pixel (pool indices as in the path kernel).
+    // Skipped when a.samples already holds its records for this frame.
     const uint32_t pixels = a.total;
-    hipError_t e = go<BLOCK, STACK, false, 1>(a, base, s);
-    if (e != hipSuccess) return e;
+    if (pass1) {
+        const hipError_t e = go<BLOCK, STACK, false, 1>(a, base, s);
+        if (e != hipSuccess) return e;
+    }
     // Pass 2: one lane per (pixel, sample).
     a.total = pixels * a.job.spp;
     return ldsgrad ? go<BLOCK, STACK, true, 2>(a, base + grad_bytes, s) : go<BLOCK, STACK, false, 2>(a, base, s);
 }
 }  // namespace
 
-hipError_t launch_adjoint(const AdjointLaunch& a, uint32_t stack, hipStream_t s)
+hipError_t launch_adjoint(const AdjointLaunch& a, uint32_t stack, bool pass1, hipStream_t s)
 {
     constexpr int kBlock = 128;
     const uint32_t np = 7u * a.job.sc.n_spheres + 1u;
     const size_t base = (size_t)((stack <= 32 ? 32 : 64) + kMaxAdjDepth) * kBlock * 4u;
     const bool lds = base + (size_t)np * 4u <= 60u * 1024u;
     const size_t grad_bytes = (size_t)np * 4u;
-    if (stack <= 32) return both_passes<kBlock, 32>(a, lds, base, grad_bytes, s);
-    return both_passes<kBlock, 64>(a, lds, base, grad_bytes, s);
+    if (stack <= 32) return both_passes<kBlock, 32>(a, pass1, lds, base, grad_bytes, s);
+    return both_passes<kBlock, 64>(a, pass1, lds, base, grad_bytes, s);
 }
 
 }  // namespace vkr

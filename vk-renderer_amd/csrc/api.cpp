@@ -259,6 +259,7 @@ vkr_error vkr_scene_upload(vkr_ctx* ctx, const vkr_scene_desc* desc, uint32_t* o
     for (int k = 0; k < 3; ++k) d.sph_c[k] = bo.sph_c[k];
     d.sph_r = bo.sph_r;
 
+    sc->serial = ctx->next_serial++;
     vkr_scene_info& in = sc->info;
     in.n_tris = desc->n_tris;
     in.n_spheres = desc->n_spheres;
@@ -315,6 +316,9 @@ vkr_error vkr_scene_set_sphere_params(vkr_ctx* ctx, uint32_t scene, const float*
     const uint32_t ns = (uint32_t)s->host_spheres.size();
     if (n != 7u * ns + 1u) return fail(VKR_INVALID_ARGUMENT, "params must hold 7 * n_spheres + 1 floats");
     VKR_HIP(hipSetDevice(ctx->device));
+    // The same values again leave the scene's version as it is (a recording
+    // forward stays valid for the adjoint that follows it).
+    const bool same = s->host_theta.size() == n && std::memcmp(s->host_theta.data(), params, (size_t)n * sizeof(float)) == 0;
     const float L = params[7u * ns];
     for (uint32_t i = 0; i < ns; ++i) {
         vkr_material& m = s->host_spheres[i].mat;
@@ -336,6 +340,7 @@ vkr_error vkr_scene_set_sphere_params(vkr_ctx* ctx, uint32_t scene, const float*
     s->info.n_emitters = (uint32_t)em.size();
     s->host_theta.assign(params, params + n);
     VKR_HIP(hipMemcpy(s->theta.p, s->host_theta.data(), (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+    if (!same) s->serial = ctx->next_serial++;
     // Captured graphs hold a copy of the scene record per job: refresh the
     // emitter count and bounds there too (the device is idle here).
     for (const auto& g : ctx->graphs) {
@@ -346,6 +351,18 @@ vkr_error vkr_scene_set_sphere_params(vkr_ctx* ctx, uint32_t scene, const float*
             VKR_HIP(hipMemcpy(dst, &s->dev, sizeof(DevScene), hipMemcpyHostToDevice));
         }
     }
+    return VKR_SUCCESS;
+}
+
+// Room for the adjoint's first-pass records of n samples in the context.
+// Growing the buffer waits for the device, as for the ray records.
+static vkr_error ensure_adj_samples(vkr_ctx* ctx, size_t n)
+{
+    const size_t bytes = n * sizeof(uint2);
+    if (ctx->adj_samples.bytes >= bytes) return VKR_SUCCESS;
+    ctx->adj_key.valid = false;
+    VKR_HIP(hipDeviceSynchronize());
+    VKR_HIP(ctx->adj_samples.alloc(bytes));
     return VKR_SUCCESS;
 }
 
@@ -374,6 +391,13 @@ vkr_error vkr_trace_frame(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd
         return fail(VKR_INVALID_ARGUMENT, "decision flags need outputs.decisions");
     if ((params->flags & (VKR_TRACE_RECORD_DECISIONS | VKR_TRACE_REPLAY_DECISIONS)) && params->max_depth > 8)
         return fail(VKR_INVALID_ARGUMENT, "decision record/replay supports max_depth <= 8");
+    const bool rec_adj = (params->flags & VKR_TRACE_RECORD_ADJOINT) != 0;
+    if (rec_adj && params->kernel != VKR_KERNEL_PATH)
+        return fail(VKR_INVALID_ARGUMENT, "VKR_TRACE_RECORD_ADJOINT needs the path kernel");
+    if (rec_adj && params->max_depth > 10)
+        return fail(VKR_INVALID_ARGUMENT, "VKR_TRACE_RECORD_ADJOINT supports max_depth <= 10 (as the adjoint)");
+    if (rec_adj && (params->flags & (VKR_TRACE_RECORD_DECISIONS | VKR_TRACE_REPLAY_DECISIONS)))
+        return fail(VKR_INVALID_ARGUMENT, "VKR_TRACE_RECORD_ADJOINT takes no decision flags (as the adjoint)");
     Scene* s = nullptr;
     if (params->kernel == VKR_KERNEL_PATH) {
         s = get_scene(ctx, scene);
@@ -385,6 +409,14 @@ vkr_error vkr_trace_frame(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd
     hipStream_t st = static_cast<hipStream_t>(stream);
     static const Scene kNoScene{};
     JobDesc j = make_job(s ? *s : kNoScene, *fd, *params, outputs);
+    if (rec_adj) {
+        // The records are rewritten from here on: they describe this frame
+        // only once its launch is in the stream.
+        ctx->adj_key.valid = false;
+        const vkr_error ae = ensure_adj_samples(ctx, (size_t)j.W * j.H * j.spp);
+        if (ae != VKR_SUCCESS) return ae;
+        j.adj_samples = ctx->adj_samples.as<uint2>();
+    }
     DevCounters* cnt = ctx->counters.as<DevCounters>();
     uint32_t* pool = ctx->pool.as<uint32_t>();
     VKR_HIP(launch_zero_counters(cnt, pool, st));
@@ -431,6 +463,7 @@ vkr_error vkr_trace_frame(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd
             VKR_HIP(launch_path(a, v, g, st));
         }
         VKR_HIP(hipEventRecord(ctx->ev1, st));
+        if (rec_adj) ctx->adj_key = adjoint_key(s->serial, *fd, *params);
     }
     ctx->timed = true;
     ctx->last_static = false;
@@ -446,6 +479,8 @@ vkr_error vkr_trace_frame_static(vkr_ctx* ctx, uint32_t scene, const vkr_frame_d
     if (!ctx || !fd || !outputs) return fail(VKR_INVALID_ARGUMENT, "NULL argument");
     const vkr_error pe = validate_params(params);
     if (pe != VKR_SUCCESS) return pe;
+    if (params->flags & VKR_TRACE_RECORD_ADJOINT)
+        return fail(VKR_INVALID_ARGUMENT, "VKR_TRACE_RECORD_ADJOINT grows a context buffer; use vkr_trace_frame");
     if ((params->flags & (VKR_TRACE_RECORD_DECISIONS | VKR_TRACE_REPLAY_DECISIONS)) && !outputs->decisions)
         return fail(VKR_INVALID_ARGUMENT, "decision flags need outputs.decisions");
     if ((params->flags & (VKR_TRACE_RECORD_DECISIONS | VKR_TRACE_REPLAY_DECISIONS)) && params->max_depth > 8)
@@ -639,18 +674,22 @@ vkr_error vkr_adjoint(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd, co
     a.counters = ctx->counters.as<DevCounters>();
     a.pool = ctx->pool.as<uint32_t>();
     a.total = a.job.n_tiles * 64u;
-    // 8 bytes per (pixel, sample) between the two passes; growing the buffer
-    // waits for the device, as for the ray records.
-    const size_t rec_bytes = (size_t)a.job.W * a.job.H * a.job.spp * sizeof(uint2);
-    if (ctx->adj_samples.bytes < rec_bytes) {
-        VKR_HIP(hipDeviceSynchronize());
-        VKR_HIP(ctx->adj_samples.alloc(rec_bytes));
+    // The first pass's records, 8 bytes per (pixel, sample), are still those
+    // of this frame when a recording vkr_trace_frame or an adjoint wrote
+    // them for the same scene version, frame data and trace parameters.
+    const AdjointKey key = adjoint_key(s->serial, *fd, *params);
+    const bool recorded = same_key(ctx->adj_key, key);
+    if (!recorded) {
+        ctx->adj_key.valid = false;
+        const vkr_error ae = ensure_adj_samples(ctx, (size_t)a.job.W * a.job.H * a.job.spp);
+        if (ae != VKR_SUCCESS) return ae;
     }
     a.samples = ctx->adj_samples.as<uint2>();
     VKR_HIP(launch_zero_counters(a.counters, a.pool, st));
     VKR_HIP(hipEventRecord(ctx->ev0, st));
-    VKR_HIP(launch_adjoint(a, s->dev.stack_needed, st));
+    VKR_HIP(launch_adjoint(a, s->dev.stack_needed, !recorded, st));
     VKR_HIP(hipEventRecord(ctx->ev1, st));
+    ctx->adj_key = key;
     ctx->timed = true;
     ctx->last_static = false;
     ctx->last_counters = nullptr;

@@ -42,6 +42,8 @@ vkr_error plan_batch(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n, BatchP
         if ((jb.params.flags & (VKR_TRACE_RECORD_DECISIONS | VKR_TRACE_REPLAY_DECISIONS)) &&
             (!jb.outputs.decisions || jb.params.max_depth > 8))
             return fail(VKR_INVALID_ARGUMENT, "decision flags need outputs.decisions and max_depth <= 8");
+        if (jb.params.flags & VKR_TRACE_RECORD_ADJOINT)
+            return fail(VKR_INVALID_ARGUMENT, "VKR_TRACE_RECORD_ADJOINT is for vkr_trace_frame only");
         Scene* s = get_scene(ctx, jb.scene);
         if (!s) return fail(VKR_INVALID_ARGUMENT, "unknown scene in batch job");
         plan.jobs[i] = make_job(*s, jb.frame_data, jb.params, &jb.outputs);

@@ -78,6 +78,7 @@ struct Scene {
     std::vector<float> host_theta;   // 7 * n_spheres + 1
     DevScene dev{};
     vkr_scene_info info{};
+    uint64_t serial = 0;             // version of the contents: new at upload and at every parameter change
 };
 
 struct Graph {
@@ -101,6 +102,38 @@ struct TuneEntry {
     uint32_t block, chunk, stack, sched;
 };
 
+// What the adjoint's first-pass records held in vkr_ctx::adj_samples were
+// computed for: the scene version and every input the paths depend on.
+struct AdjointKey {
+    bool valid = false;
+    uint64_t serial = 0;
+    vkr_frame_data fd{};
+    uint32_t width = 0, height = 0, spp = 0, max_depth = 0;
+    float epsilon = 0.0f;
+};
+
+inline AdjointKey adjoint_key(uint64_t serial, const vkr_frame_data& fd, const vkr_trace_params& p)
+{
+    AdjointKey k;
+    k.valid = true;
+    k.serial = serial;
+    k.fd = fd;
+    k.width = p.width;
+    k.height = p.height;
+    k.spp = p.spp;
+    k.max_depth = p.max_depth;
+    k.epsilon = p.epsilon;
+    return k;
+}
+
+inline bool same_key(const AdjointKey& a, const AdjointKey& b)
+{
+    // Bytewise on the frame data: padding that differs only costs a first pass.
+    return a.valid && b.valid && a.serial == b.serial && std::memcmp(&a.fd, &b.fd, sizeof(a.fd)) == 0 &&
+           a.width == b.width && a.height == b.height && a.spp == b.spp && a.max_depth == b.max_depth &&
+           std::memcmp(&a.epsilon, &b.epsilon, sizeof(float)) == 0;
+}
+
 // Autotuned launch shape for one batch job set (batch.cpp: batch_key).
 struct BatchTuneEntry {
     uint64_t key;
@@ -119,6 +152,8 @@ struct vkr_ctx {
     vkr::DevBuf lanes;                 // per-lane ray records of direct launches (grown on demand)
     vkr::DevBuf build_scratch;         // device BVH builds (grown on demand, reused across uploads)
     vkr::DevBuf adj_samples;           // the adjoint's per-sample records (grown on demand)
+    vkr::AdjointKey adj_key;           // what adj_samples holds (valid: a later adjoint may skip its first pass)
+    uint64_t next_serial = 1;          // source of Scene::serial
     void* host_jobs = nullptr;         // pinned staging for batched job tables
     size_t host_jobs_bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_copy = nullptr;

@@ -50,6 +50,8 @@ struct JobDesc {
     float* hit_t;
     uint32_t* ray_count;
     uint16_t* decisions;
+    uint2* adj_samples;           // W*H*spp, pixel-major: (bits of the random-stream position at the
+                                  // sample's start, clamp-saturation bits) for the adjoint; NULL: none
 };
 
 struct DevCounters {
@@ -136,7 +138,8 @@ hipError_t launch_path(const PathLaunch& a, const PathVariant& v, uint32_t grid,
 hipError_t launch_whitted(const JobDesc& j, DevCounters* c, hipStream_t s);
 hipError_t launch_filter(const uint8_t* cur, const uint8_t* prev, uint32_t sw, uint32_t sh, uint8_t* out,
                          uint32_t dw, uint32_t dh, hipStream_t s);
-hipError_t launch_adjoint(const AdjointLaunch& a, uint32_t stack, hipStream_t s);
+// pass1 false: a.samples already holds the first pass's records of this frame.
+hipError_t launch_adjoint(const AdjointLaunch& a, uint32_t stack, bool pass1, hipStream_t s);
 hipError_t launch_debug(uint32_t op, const float* in, float* out, uint32_t n, hipStream_t s);
 hipError_t launch_zero_counters(DevCounters* c, uint32_t* pool, hipStream_t s);
 // Writes the per-frame fields of a job in a device job table (camera,

@@ -277,6 +277,9 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
             drep = 0;
             if ((j.flags & VKR_TRACE_REPLAY_DECISIONS) && j.decisions)
                 drep = j.decisions[((size_t)py * j.W + px) * j.spp + smp];
+            // The adjoint's first pass, kept on request: where this sample's
+            // random stream starts (its clamp bits follow, per bounce).
+            if (j.adj_samples) j.adj_samples[((size_t)py * j.W + px) * j.spp + smp] = make_uint2(__float_as_uint(salt), 0u);
         }
 
         // This step's rays: with POOL up to kRaySlots in the lane's record,
@@ -288,6 +291,14 @@ __device__ __forceinline__ void path_program(const PathLaunch& A)
         float t0max = 0.0f;
         if (has) {
             if (phase == kPhTrace) {
+                if (j.adj_samples) {
+                    // Channels that clamp(acc) saturates at this bounce, 10 bits per channel.
+                    const uint32_t bit = 1u << depth;
+                    const uint32_t zb = ((acc.x > 1.0f || acc.x < 0.0f) ? bit : 0u) |
+                                        ((acc.y > 1.0f || acc.y < 0.0f) ? bit << 10 : 0u) |
+                                        ((acc.z > 1.0f || acc.z < 0.0f) ? bit << 20 : 0u);
+                    if (zb) j.adj_samples[((size_t)py * j.W + px) * j.spp + smp].y |= zb;
+                }
                 acc = clamps(acc, 0.0f, 1.0f);                      // :441
                 const float dp1 = (float)(depth + 1u);
                 const float tmax = 3000.0f / (dp1 * dp1);            // :444

@@ -4,8 +4,11 @@
     radiance = render(theta, ctx, sid, workload)          # (H*W, 3) on cuda:0
     loss = ((radiance - target) ** 2).sum(); loss.backward()
 
-Forward sets the scene's sphere parameters and runs the path kernel; backward
-runs the adjoint kernel, which replays every path from its RNG state.  Branch
+Forward sets the scene's sphere parameters and runs the path kernel, which
+also records where each sample's random stream starts (and its clamp bits);
+backward runs the adjoint kernel, which replays every path from its RNG state
+(the adjoint recomputes the records itself when the scene, its parameters or
+the frame changed in between).  Branch
 decisions (emitters, Russian roulette, dielectric choice) are held fixed, so
 the gradient is that of the per-sample estimate.
 """
@@ -24,7 +27,13 @@ class _Render(torch.autograd.Function):
         vctx.set_sphere_params(sid, host)
         n = params.width * params.height
         rad = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
-        vctx.trace(sid, frame, params, radiance=rad)
+        fwd = params
+        if fctx.needs_input_grad[0] and params.max_depth <= 10 and params.kernel == vkr.KERNEL_PATH and not (
+                params.flags & (vkr.TRACE_RECORD_DECISIONS | vkr.TRACE_REPLAY_DECISIONS)):
+            # Keep the adjoint's first pass: backward then runs only its second.
+            fwd = vkr.TraceParams.from_buffer_copy(params)
+            fwd.flags |= vkr.TRACE_RECORD_ADJOINT
+        vctx.trace(sid, frame, fwd, radiance=rad)
         fctx.vctx, fctx.sid, fctx.frame, fctx.params = vctx, sid, frame, params
         fctx.theta_host = host
         fctx.theta_device = theta.device

@@ -40,6 +40,7 @@ KERNEL_PATH, KERNEL_WHITTED = 0, 1
 SCHED_AUTO, SCHED_PLAIN, SCHED_PERSISTENT = 0, 1, 2
 TRACE_RECORD_DECISIONS, TRACE_REPLAY_DECISIONS = 1, 2
 TRACE_RAY_POOL, TRACE_NO_RAY_POOL = 4, 8
+TRACE_RECORD_ADJOINT = 16
 SCENE_TRI_MATERIAL_INDEX = 1
 SCENE_DEVICE_BVH = 2
 BUILDER_HOST_SAH, BUILDER_DEVICE_LBVH = 0, 1
