@@ -159,7 +159,7 @@ enum {
      * instead of the host SAH builder: uploads of large scenes take
      * milliseconds, traversal is a few percent slower, images are the same.
      * traversal_cost does not apply; when the tree is deeper than max_depth
-     * (0 = 64) the host builder is used instead (vkr_scene_info.builder says
+     * (0 = 63) the host builder is used instead (vkr_scene_info.builder says
      * which ran).  Build extension: the reference has no acceleration
      * structure. */
     VKR_SCENE_DEVICE_BVH = 1u << 1
@@ -260,10 +260,11 @@ enum {
      * These force one or the other; the results are identical either way. */
     VKR_TRACE_RAY_POOL         = 1u << 2,
     VKR_TRACE_NO_RAY_POOL      = 1u << 3,
-    /* vkr_trace_frame only (max_depth <= 10, no decision replay): also keeps,
-     * in the context, what the first pass of vkr_adjoint computes for this
-     * frame (each sample's random-stream position at its start and which
-     * channels of its clamp saturate, 8 bytes per sample).  The next
+    /* vkr_trace_frame only (path kernel, max_depth <= 10, no decision
+     * flags): also keeps, in the context, what the first pass of vkr_adjoint
+     * computes for this frame (each sample's random-stream position at its
+     * start and which channels of its clamp saturate, 8 bytes per sample;
+     * growing that buffer waits for the device).  The next
      * vkr_adjoint with the same scene, sphere parameters, frame data and
      * trace parameters then runs only its second pass.  The image is
      * unchanged.  Refused by vkr_trace_batch, vkr_batch_graph_create and
@@ -377,7 +378,9 @@ vkr_error vkr_autotune_batch(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n
  * VKR_TRACE_RECORD_ADJOINT or by the previous vkr_adjoint.  Those records
  * are read in stream order: run the recording call on the same stream or
  * order the streams.  Branch decisions (emitter set, Russian roulette,
- * dielectric choice) are held fixed. */
+ * dielectric choice) are held fixed.  At most 0xFF000000 (pixel, sample)
+ * pairs per call, pixels counted over whole 8x8 tiles (VKR_UNSUPPORTED
+ * beyond). */
 vkr_error vkr_adjoint(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* frame_data,
                       const vkr_trace_params* params, const float* d_radiance,
                       float* d_params, void* stream);

@@ -205,7 +205,7 @@ def test_adjoint_is_linear_in_the_output_gradient(ctx, torch_cuda):
     """At the full adjoint workload (256^2, 16 spp, 4 bounces): the parameter
     gradient is linear in the image gradient, adjoint(2 g1) = 2 adjoint(g1)
     and adjoint(g1 + g2) = adjoint(g1) + adjoint(g2), up to the order of the
-    float additions that reduce it."""
+    float additions that reduce it (which varies between calls)."""
     torch = torch_cuda
     wl = scenes.adjoint_scene()
     sid = ctx.upload(wl.scene)
@@ -220,13 +220,18 @@ def test_adjoint_is_linear_in_the_output_gradient(ctx, torch_cuda):
         torch.cuda.synchronize()
         return d.double().cpu().numpy()
 
-    a1, a2, a12, a1x2 = adj(g1), adj(g2), adj(g1 + g2), adj(2.0 * g1)
+    a1, a2, a12, a1x2, a1b = adj(g1), adj(g2), adj(g1 + g2), adj(2.0 * g1), adj(g1)
     scale = np.linalg.norm(a1) + np.linalg.norm(a2)
     e_scale = np.linalg.norm(a1x2 - 2.0 * a1) / (2.0 * np.linalg.norm(a1))
     e_sum = np.linalg.norm(a12 - a1 - a2) / scale
-    print(f"\n[adjoint linearity] |a1| {np.linalg.norm(a1):.4g} scale err {e_scale:.3g} sum err {e_sum:.3g}")
+    # The same call twice differs only in the order of the float atomics
+    # that reduce the gradient (1e-7..2e-6 relative here): the bound for the
+    # two identities sits above that noise.
+    e_rep = np.linalg.norm(a1b - a1) / np.linalg.norm(a1)
+    print(f"\n[adjoint linearity] |a1| {np.linalg.norm(a1):.4g} scale err {e_scale:.3g} sum err {e_sum:.3g} "
+          f"repeat {e_rep:.3g}")
     assert np.linalg.norm(a1) > 0
-    assert e_scale <= 1e-6 and e_sum <= 1e-5
+    assert e_rep <= 1e-5 and e_scale <= 1e-5 and e_sum <= 1e-5
     ctx.destroy_scene(sid)
 
 
@@ -482,4 +487,19 @@ def test_recording_flag_refusals(ctx, torch_cuda):
             fn()
     ctx.trace(sid, wl.frame, wl.params(flags=R), radiance=rad)
     torch.cuda.synchronize()
+    ctx.destroy_scene(sid)
+
+
+def test_adjoint_size_limit(ctx, torch_cuda):
+    """The second pass hands out one item per (pixel, sample) from a 32-bit
+    cursor: 4096^2 pixels at 256 spp (2^32 pairs) are refused before any
+    allocation or launch; 4096^2 at 255 spp would pass the check."""
+    torch = torch_cuda
+    wl = scenes.adjoint_scene(width=8, height=8, spp=1)
+    sid = ctx.upload(wl.scene)
+    g4 = torch.zeros((64, 4), dtype=torch.float32, device="cuda")
+    d = torch.zeros(7 * len(wl.scene.spheres) + 1, dtype=torch.float32, device="cuda")
+    with pytest.raises(vkr.VkrError) as err:
+        ctx.adjoint(sid, wl.frame, vkr.Context.params(4096, 4096, 256, 4), g4, d)
+    assert err.value.code == 7 and "0xFF000000" in str(err.value)   # VKR_UNSUPPORTED
     ctx.destroy_scene(sid)

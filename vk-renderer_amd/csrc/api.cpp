@@ -674,6 +674,9 @@ vkr_error vkr_adjoint(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd, co
     a.counters = ctx->counters.as<DevCounters>();
     a.pool = ctx->pool.as<uint32_t>();
     a.total = a.job.n_tiles * 64u;
+    // The second pass hands out one item per (pixel, sample) of whole tiles.
+    if ((uint64_t)a.total * a.job.spp > kMaxPoolItems)
+        return fail(VKR_UNSUPPORTED, "the adjoint takes at most 0xFF000000 (pixel, sample) pairs per call (8x8 tiles)");
     // The first pass's records, 8 bytes per (pixel, sample), are still those
     // of this frame when a recording vkr_trace_frame or an adjoint wrote
     // them for the same scene version, frame data and trace parameters.

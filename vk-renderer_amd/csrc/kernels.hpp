@@ -64,6 +64,12 @@ struct DevCounters {
 // 1 refills too often).
 constexpr uint32_t kDefaultChunk = 8u;
 
+// Most work items one launch's 32-bit pool cursor hands out.  A resident
+// wave can add up to 64 to the cursor after the last item has gone (its one
+// claim that finds the pool empty), so the cursor stays below 2^32 while the
+// resident lanes number under 2^24.
+constexpr uint64_t kMaxPoolItems = 0xFF000000ull;
+
 // One-pixel-per-lane launches cover at most this many lanes each, which
 // bounds their ray records (2^21 x kLaneBytes = 2.7 GB).
 constexpr uint32_t kMaxPlainLanes = 1u << 21;
