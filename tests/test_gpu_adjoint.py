@@ -563,3 +563,70 @@ def test_adjoint_triangle_materials_match_finite_differences(ctx, torch_cuda):
         print(f"  param {idx[k]}: rel {rel[k]:.3g} at h=1e-3 -> {r4:.3g} at h=2.5e-4")
         assert r4 < rel[k] and r4 <= 5e-3
     ctx.destroy_scene(sid)
+
+
+def test_adjoint_dielectric_and_emissive_spheres_match_finite_differences(ctx, torch_cuda):
+    """Spheres of all four kinds, two dielectrics among them also emitting
+    (then they are next-event emitters too, and their emission enters the
+    path unnormalised, Tracer.comp:515): albedos, the emitters' emission,
+    the light scale and the roughness of diffuse and metal spheres against
+    central differences of the HIP forward with frozen decisions (positive
+    loss weights).  A dielectric's roughness holds its IOR, which moves the
+    refracted ray and is not differentiated; emission parameters of spheres
+    that do not emit would change the emitter set, a jump."""
+    torch = torch_cuda
+    w, h, spp = 96, 64, 8
+    rng = np.random.default_rng(33)
+    sph = scenes.random_spheres(rng, 24, -10.0, 10.0)
+    diel = [i for i in range(len(sph)) if sph[i]["mat"]["type"] == vkr.DIELECTRIC]
+    assert len(diel) >= 3
+    # Two of them emit, placed in front of the camera (at z = -30, looking
+    # along +z) so that many paths pass through them.
+    for i, c in zip(diel[:2], ((-3.0, 0.0, -8.0), (4.0, -2.0, -4.0))):
+        sph[i]["mat"]["emissive"] = np.array([3.0, 2.0, 1.5], np.float32)
+        sph[i]["P"] = np.array(c, np.float32)
+        sph[i]["r"] = np.float32(2.0)
+    sph = np.concatenate([sph, scenes.light_sphere((0.0, 30.0, 0.0), 6.0)])
+    wl0 = scenes.adjoint_scene(width=w, height=h, spp=spp)
+    sc = vkr.Scene(spheres=sph, planes=wl0.scene.planes)
+    wl = scenes.Workload("adjoint_kinds", sc, wl0.frame, w, h, spp, wl0.max_depth)
+    sid = ctx.upload(sc)
+    theta = scenes.sphere_params(sc.spheres, light_scale=1.0)
+    dec = torch.zeros(w * h * spp, dtype=torch.int16, device="cuda")
+    base = _forward(ctx, torch, sid, wl, theta, vkr.TRACE_RECORD_DECISIONS, dec)
+    weights = np.random.default_rng(6).uniform(0.25, 1.0, base.shape)
+    g4 = torch.zeros((w * h, 4), dtype=torch.float32, device="cuda")
+    g4[:, :3] = torch.from_numpy(weights.astype(np.float32)).cuda()
+    d = torch.zeros(theta.size, dtype=torch.float32, device="cuda")
+    ctx.set_sphere_params(sid, theta)
+    ctx.adjoint(sid, wl.frame, wl.params(), g4, d)
+    torch.cuda.synchronize()
+    g = d.double().cpu().numpy()
+    n_sph = len(sph)
+    emitting = [s for s in range(n_sph) if np.any(sph[s]["mat"]["emissive"] != 0)]
+    idx = [7 * s + k for s in range(n_sph) for k in (0, 1, 2)]
+    idx += [7 * s + 6 for s in range(n_sph) if sph[s]["mat"]["type"] != vkr.DIELECTRIC]
+    idx += [7 * s + k for s in emitting for k in (3, 4, 5)] + [7 * n_sph]
+    fd = np.array([_fd(ctx, torch, sid, wl, theta, dec, i, weights) for i in idx])
+    ga = g[idx]
+    scale = np.abs(fd).max()
+    big = np.abs(fd) > 1e-3 * scale
+    agg = np.linalg.norm(ga[big] - fd[big]) / np.linalg.norm(fd[big])
+    rel = np.abs(ga - fd) / np.maximum(np.abs(fd), 1e-3 * scale)
+    print(f"\n[adjoint sphere kinds] {len(emitting)} emitters, {big.sum()} parameters, aggregate relative error "
+          f"{agg:.3g}, {(rel[big] <= 1e-3).mean() * 100:.1f}% within 1e-3")
+    for k in np.argsort(-rel)[:4]:
+        print(f"  param {idx[k]:4d} adjoint {ga[k]: .6e} fd {fd[k]: .6e} rel {rel[k]:.3g}")
+    assert big.sum() >= 30
+    # the emitting dielectrics' own emission gradients are in the check
+    for s in diel[:2]:
+        print(f"  emitting dielectric {s}: emission fd {[round(float(fd[idx.index(7 * s + k)]), 5) for k in (3, 4, 5)]}")
+        assert all(big[idx.index(7 * s + k)] for k in (3, 4, 5))
+    assert agg <= 1e-3
+    assert (rel[big] <= 1e-3).mean() >= 0.9
+    for k in np.where(big & (rel > 1e-3))[0]:
+        f4 = _fd(ctx, torch, sid, wl, theta, dec, idx[k], weights, h=2.5e-4)
+        r4 = abs(ga[k] - f4) / abs(f4)
+        print(f"  param {idx[k]}: rel {rel[k]:.3g} at h=1e-3 -> {r4:.3g} at h=2.5e-4")
+        assert r4 < rel[k] and r4 <= 5e-3
+    ctx.destroy_scene(sid)
