@@ -630,3 +630,64 @@ def test_adjoint_dielectric_and_emissive_spheres_match_finite_differences(ctx, t
         print(f"  param {idx[k]}: rel {rel[k]:.3g} at h=1e-3 -> {r4:.3g} at h=2.5e-4")
         assert r4 < rel[k] and r4 <= 5e-3
     ctx.destroy_scene(sid)
+
+
+def test_adjoint_depth8_matches_finite_differences(ctx, torch_cuda):
+    """Eight bounces, the deepest for which decisions can be frozen
+    (recorded for depths 0-7): the saturation bits and the throughput chain
+    of long paths against central differences of the HIP forward (positive
+    loss weights).  The spheres' albedos are raised and the scene is closed
+    by planes, so that paths reach the last bounces."""
+    torch = torch_cuda
+    w, h, spp = 96, 64, 8
+    wl0 = scenes.adjoint_scene(width=w, height=h, spp=spp, max_depth=8)
+    sph = wl0.scene.spheres.copy()
+    for i in range(len(sph) - 1):
+        sph[i]["mat"]["albedo"] = np.float32(0.5) + np.float32(0.45) * sph[i]["mat"]["albedo"]
+    # A closed room of bright diffuse planes (floor, ceiling, four walls;
+    # dot(X, N) + len = 0) keeps paths inside it.
+    room = [((0, 1, 0), 11.0), ((0, -1, 0), 40.0), ((1, 0, 0), 25.0), ((-1, 0, 0), 25.0),
+            ((0, 0, -1), 25.0), ((0, 0, 1), 35.0)]
+    pln = scenes.planes_from([vkr.material((0.85, 0.8, 0.75), (0, 0, 0), 0.5, 0.0) for _ in room],
+                             [n for n, _ in room], np.array([l for _, l in room], np.float32))
+    sc = vkr.Scene(spheres=sph, planes=pln)
+    wl = scenes.Workload("adjoint_depth8", sc, wl0.frame, w, h, spp, 8)
+    sid = ctx.upload(sc)
+    theta = scenes.sphere_params(sc.spheres, light_scale=1.0)
+    dec = torch.zeros(w * h * spp, dtype=torch.int16, device="cuda")
+    base = _forward(ctx, torch, sid, wl, theta, vkr.TRACE_RECORD_DECISIONS, dec)
+    # samples that survive Russian roulette at bounce k (decision bit 2k)
+    bits = dec.cpu().numpy().view(np.uint16)
+    surv = [int((bits >> (2 * k) & 1).sum()) for k in range(8)]
+    late = surv[5]
+    weights = np.random.default_rng(8).uniform(0.25, 1.0, base.shape)
+    g4 = torch.zeros((w * h, 4), dtype=torch.float32, device="cuda")
+    g4[:, :3] = torch.from_numpy(weights.astype(np.float32)).cuda()
+    d = torch.zeros(theta.size, dtype=torch.float32, device="cuda")
+    ctx.set_sphere_params(sid, theta)
+    ctx.adjoint(sid, wl.frame, wl.params(), g4, d)
+    torch.cuda.synchronize()
+    g = d.double().cpu().numpy()
+    n_sph = len(sph)
+    idx = [7 * s + k for s in range(n_sph - 1) for k in (0, 1, 2, 6)]
+    idx += [7 * (n_sph - 1) + k for k in (3, 4, 5)] + [7 * n_sph]
+    fd = np.array([_fd(ctx, torch, sid, wl, theta, dec, i, weights) for i in idx])
+    ga = g[idx]
+    scale = np.abs(fd).max()
+    big = np.abs(fd) > 1e-3 * scale
+    agg = np.linalg.norm(ga[big] - fd[big]) / np.linalg.norm(fd[big])
+    rel = np.abs(ga - fd) / np.maximum(np.abs(fd), 1e-3 * scale)
+    print(f"\n[adjoint depth 8] samples surviving roulette per bounce {surv}, {big.sum()} parameters, "
+          f"aggregate relative error {agg:.3g}, {(rel[big] <= 1e-3).mean() * 100:.1f}% within 1e-3")
+    assert late > 500
+    assert big.sum() >= 30
+    assert agg <= 1e-3
+    # Long paths straddle more clamp saturations at h = 1e-3 (90% of the 41
+    # parameters within 1e-3 here); each outlier must close in at h / 4.
+    assert (rel[big] <= 1e-3).mean() >= 0.85
+    for k in np.where(big & (rel > 1e-3))[0]:
+        f4 = _fd(ctx, torch, sid, wl, theta, dec, idx[k], weights, h=2.5e-4)
+        r4 = abs(ga[k] - f4) / abs(f4)
+        print(f"  param {idx[k]}: rel {rel[k]:.3g} at h=1e-3 -> {r4:.3g} at h=2.5e-4")
+        assert r4 < rel[k] and r4 <= 5e-3
+    ctx.destroy_scene(sid)
