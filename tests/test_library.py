@@ -103,3 +103,40 @@ def test_sphere_params_round_trip():
     p = scenes.sphere_params(wl.scene.spheres, light_scale=128.0)
     assert p.size == 7 * len(wl.scene.spheres) + 1
     assert np.allclose(p[7 * 32 + 3:7 * 32 + 6] * p[-1], 128.0)
+
+
+def _header_enums():
+    """Every `VKR_NAME = value` of include/vkr.h, values evaluated (integers,
+    `1u << k`)."""
+    import re
+    text = open(vkr.HEADER_PATH).read()
+    out = {}
+    for name, expr in re.findall(r"\b(VKR_[A-Z0-9_]+)\s*=\s*([0-9u<\s]+?)\s*(?=[,}/\n])", text):
+        out[name] = eval(expr.replace("u", ""), {})
+    return out
+
+
+def test_python_constants_mirror_the_header():
+    """vkr.py's flag and enum constants are the header's values (a caller
+    through ctypes passes them across the ABI as they are)."""
+    enums = _header_enums()
+    mirrored = {
+        "DIFFUSE": "VKR_MAT_DIFFUSE", "DIELECTRIC": "VKR_MAT_DIELECTRIC",
+        "KERNEL_PATH": "VKR_KERNEL_PATH", "KERNEL_WHITTED": "VKR_KERNEL_WHITTED",
+        "SCHED_AUTO": "VKR_SCHED_AUTO", "SCHED_PLAIN": "VKR_SCHED_PLAIN", "SCHED_PERSISTENT": "VKR_SCHED_PERSISTENT",
+        "TRACE_RECORD_DECISIONS": "VKR_TRACE_RECORD_DECISIONS",
+        "TRACE_REPLAY_DECISIONS": "VKR_TRACE_REPLAY_DECISIONS",
+        "TRACE_RAY_POOL": "VKR_TRACE_RAY_POOL", "TRACE_NO_RAY_POOL": "VKR_TRACE_NO_RAY_POOL",
+        "TRACE_RECORD_ADJOINT": "VKR_TRACE_RECORD_ADJOINT",
+        "SCENE_TRI_MATERIAL_INDEX": "VKR_SCENE_TRI_MATERIAL_INDEX", "SCENE_DEVICE_BVH": "VKR_SCENE_DEVICE_BVH",
+        "BUILDER_HOST_SAH": "VKR_BUILDER_HOST_SAH", "BUILDER_DEVICE_LBVH": "VKR_BUILDER_DEVICE_LBVH",
+        "CTX_COUNT_WORK": "VKR_CTX_COUNT_WORK",
+    }
+    for py, c in mirrored.items():
+        assert c in enums, c
+        assert getattr(vkr, py) == enums[c], (py, getattr(vkr, py), enums[c])
+    for op, v in vkr.DEBUG_OPS.items():
+        assert enums["VKR_DEBUG_" + op.upper()] == v
+    # every VKR_TRACE_* flag has its Python name
+    flags = {k for k in enums if k.startswith("VKR_TRACE_")}
+    assert flags == {c for c in mirrored.values() if c.startswith("VKR_TRACE_")}
