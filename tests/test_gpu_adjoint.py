@@ -691,3 +691,46 @@ def test_adjoint_depth8_matches_finite_differences(ctx, torch_cuda):
         print(f"  param {idx[k]}: rel {rel[k]:.3g} at h=1e-3 -> {r4:.3g} at h=2.5e-4")
         assert r4 < rel[k] and r4 <= 5e-3
     ctx.destroy_scene(sid)
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_recording_on_random_scenes(ctx, torch_cuda, seed):
+    """VKR_TRACE_RECORD_ADJOINT over the random scenes of the parity sweep
+    (all four material kinds, per-triangle material tables, planes, odd
+    sizes, 1-6 bounces, several epsilons): every forward output is
+    bit-identical with and without the flag; the adjoint after it traces
+    exactly half the rays of the adjoint on its own (pass 2 replays pass 1's
+    rays) and gives the same gradient up to the order of its float atomics."""
+    import test_gpu_sweep
+    torch = torch_cuda
+    wl, eps = test_gpu_sweep._scene(seed)
+    sid = ctx.upload(wl.scene)
+    n = wl.width * wl.height
+    p = vkr.Context.params(wl.width, wl.height, wl.spp, wl.max_depth, epsilon=eps)
+    rec = vkr.Context.params(wl.width, wl.height, wl.spp, wl.max_depth, epsilon=eps,
+                             flags=vkr.TRACE_RECORD_ADJOINT)
+    outs = []
+    for params in (p, rec):
+        o = {"rgba8": torch.zeros((n, 4), dtype=torch.uint8, device="cuda"),
+             "radiance": torch.zeros((n, 4), dtype=torch.float32, device="cuda"),
+             "prim_id": torch.zeros(n, dtype=torch.int32, device="cuda"),
+             "hit_t": torch.zeros(n, dtype=torch.float32, device="cuda"),
+             "ray_count": torch.zeros(n, dtype=torch.int32, device="cuda")}
+        ctx.trace(sid, wl.frame, params, **o)
+        torch.cuda.synchronize()
+        outs.append({k: v.cpu().numpy().view(np.uint8) for k, v in o.items()})
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+    n_par = 7 * len(wl.scene.spheres) + 1
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    g4 = (torch.rand((n, 4), device="cuda", generator=gen) - 0.5).contiguous()
+    _adjoint(ctx, torch, sid, _other(wl.frame), p, g4, n_par)
+    g_own, rays_own = _adjoint(ctx, torch, sid, wl.frame, p, g4, n_par)
+    ctx.trace(sid, wl.frame, rec, radiance=torch.zeros((n, 4), dtype=torch.float32, device="cuda"))
+    g_rec, rays_rec = _adjoint(ctx, torch, sid, wl.frame, p, g4, n_par)
+    diff = np.linalg.norm(g_rec - g_own)
+    print(f"\n[recording sweep {seed}] {wl.width}x{wl.height} spp {wl.spp} depth {wl.max_depth}: rays {rays_own} "
+          f"-> {rays_rec}, |g| {np.linalg.norm(g_own):.4g}, difference {diff:.3g}")
+    assert rays_own == 2 * rays_rec
+    assert diff <= 1e-5 * np.linalg.norm(g_own)
+    ctx.destroy_scene(sid)
