@@ -819,3 +819,42 @@ def test_adjoint_random_scenes_match_finite_differences(ctx, torch_cuda, seed):
         print(f"  param {idx[k]}: rel {rel[k]:.3g} at h=1e-3, {r4:.3g} at h=2.5e-4, {r16:.3g} at h=6.25e-5")
         assert min(r4, r16) <= 5e-3
     ctx.destroy_scene(sid)
+
+
+def test_diff_render_backward_runs_pass_two_only(ctx, torch_cuda):
+    """diff.render's forward passes VKR_TRACE_RECORD_ADJOINT when theta needs
+    a gradient, and its backward (which sets the same sphere parameters
+    again) then runs only the adjoint's second pass: half the rays of the
+    adjoint on its own, and the same gradient.  Without a gradient the
+    forward records nothing."""
+    torch = torch_cuda
+    import diff
+    wl = scenes.adjoint_scene(width=96, height=64, spp=4)
+    sid = ctx.upload(wl.scene)
+    theta = scenes.sphere_params(wl.scene.spheres, light_scale=1.5)
+    n = wl.width * wl.height
+    gen = torch.Generator(device="cuda").manual_seed(2)
+    w = torch.rand((n, 3), device="cuda", generator=gen)
+    # the adjoint on its own (the previous call was for another frame)
+    g4 = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+    g4[:, :3] = w
+    ctx.set_sphere_params(sid, theta)
+    _adjoint(ctx, torch, sid, _other(wl.frame), wl.params(), g4.contiguous(), theta.size)
+    g_own, rays_own = _adjoint(ctx, torch, sid, wl.frame, wl.params(), g4.contiguous(), theta.size)
+    # forward + backward through torch.autograd
+    t = torch.tensor(theta, device="cuda", requires_grad=True)
+    rad = diff.render(t, ctx, sid, wl.frame, wl.params())
+    (rad * w).sum().backward()
+    torch.cuda.synchronize()
+    rays_bwd = ctx.counters()["rays"]
+    g = t.grad.double().cpu().numpy()
+    print(f"\n[diff.render] rays: adjoint on its own {rays_own}, backward {rays_bwd}")
+    assert rays_own == 2 * rays_bwd
+    assert np.linalg.norm(g - g_own) <= 1e-5 * np.linalg.norm(g_own)
+    # no gradient wanted: no recording, so the next adjoint runs both passes
+    _adjoint(ctx, torch, sid, _other(wl.frame), wl.params(), g4.contiguous(), theta.size)
+    with torch.no_grad():
+        diff.render(torch.tensor(theta, device="cuda"), ctx, sid, wl.frame, wl.params())
+    _, rays = _adjoint(ctx, torch, sid, wl.frame, wl.params(), g4.contiguous(), theta.size)
+    assert rays == rays_own
+    ctx.destroy_scene(sid)
