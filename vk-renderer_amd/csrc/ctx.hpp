@@ -78,7 +78,7 @@ struct Scene {
     std::vector<float> host_theta;   // 7 * n_spheres + 1
     DevScene dev{};
     vkr_scene_info info{};
-    uint64_t serial = 0;             // version of the contents: new at upload and at every parameter change
+    uint64_t serial = 0;             // version of the contents: new at upload and whenever a parameter value changes
 };
 
 struct Graph {
