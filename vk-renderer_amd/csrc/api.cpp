@@ -367,17 +367,26 @@ static vkr_error ensure_adj_samples(vkr_ctx* ctx, size_t n)
 }
 
 // Trees of depth <= 23 can take the 24-entry-stack kernel, which keeps 5
-// waves per SIMD resident (4 with deeper stacks) but spills more in its
-// per-step code.  The extra wave pays off only when the image has well over
-// the pixels the 4-wave kernel keeps resident (1080p soup frame: 90 ms
-// against 95 ms; a 512 x 512 frame, which the 4-wave grid holds at once, 7%
-// slower), so untuned smaller frames take the 32-entry kernel.
-static uint32_t default_stack(const PathVariant& v, uint32_t total, int device)
+// waves per SIMD resident (4 with the 32-entry stack, 2 to 3 with the
+// 64-entry one, both with more registers and fewer spills).  Untuned
+// launches choose by the image size against the lanes the 4-wave kernel
+// keeps resident, and by the tree's depth (autotuner tables, one MI355X):
+// - trees deeper than 8 levels, where node fetches dominate: the fifth wave
+//   pays off only with well over twice those pixels (1080p soup frame: 89
+//   against 95 ms; a 512 x 512 soup frame 1% slower), else the 32-entry
+//   kernel;
+// - shallow trees, where shading dominates: the 32-entry kernel (1024^2
+//   reference scene: 2.02 against 2.15 ms, 2048^2: 7.0 against 7.6 ms), or
+//   the 64-entry one when the grid holds every pixel at once (512^2 with 64
+//   spheres: 0.61 against 0.66 ms; 256^2 adjoint scene: 1.43 against
+//   1.48 ms).
+static uint32_t default_stack(const PathVariant& v, uint32_t total, uint32_t depth, int device)
 {
     if (v.stack != 24u) return v.stack;
     PathVariant v4 = v;
     v4.stack = 32u;
     const uint64_t resident4 = (uint64_t)persistent_grid(v4, device) * v.block;
+    if (depth <= 8u) return (uint64_t)total <= resident4 ? 64u : 32u;
     return (uint64_t)total > 2u * resident4 ? 24u : 32u;
 }
 
@@ -445,7 +454,7 @@ vkr_error vkr_trace_frame(vkr_ctx* ctx, uint32_t scene, const vkr_frame_data* fd
         a.chunk = chunk ? chunk : kDefaultChunk;
         PathVariant v{block, stack, (ctx->flags & VKR_CTX_COUNT_WORK) != 0, false,
                       use_ray_pool(params->flags, s->dev.n_emitters)};
-        if (!tuned) v.stack = default_stack(v, a.total, ctx->device);
+        if (!tuned) v.stack = default_stack(v, a.total, s->info.depth, ctx->device);
         // Persistent: one resident grid.  One pixel per lane: launches of at
         // most kMaxPlainLanes lanes each, in pool order.
         const uint32_t step = persistent ? a.total : kMaxPlainLanes;
@@ -525,7 +534,7 @@ vkr_error vkr_trace_frame_static(vkr_ctx* ctx, uint32_t scene, const vkr_frame_d
         PathVariant v{block, stack, (ctx->flags & VKR_CTX_COUNT_WORK) != 0, false,
                       use_ray_pool(params->flags, s->dev.n_emitters)};
         if (!tuned) {
-            v.stack = default_stack(v, a.total, ctx->device);
+            v.stack = default_stack(v, a.total, s->info.depth, ctx->device);
             // The static records cap the pooled grid below what even 4 waves
             // keep resident; then the fifth wave cannot help.
             PathVariant v4 = v;
