@@ -24,6 +24,7 @@ struct BatchPlan {
     std::vector<JobDesc> jobs;
     std::vector<uint32_t> start;  // n + 1 pool offsets
     uint32_t stack = 24;          // the smallest stack variant; raised to the jobs' needs
+    uint32_t tree_depth = 0;      // the deepest of the jobs' trees
     uint64_t total = 0;
     bool pool = false;            // some job wants the pooled kernel
 };
@@ -48,6 +49,7 @@ vkr_error plan_batch(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n, BatchP
         if (!s) return fail(VKR_INVALID_ARGUMENT, "unknown scene in batch job");
         plan.jobs[i] = make_job(*s, jb.frame_data, jb.params, &jb.outputs);
         plan.stack = std::max(plan.stack, s->dev.stack_needed);
+        plan.tree_depth = std::max(plan.tree_depth, s->info.depth);
         plan.pool = plan.pool || use_ray_pool(jb.params.flags, s->dev.n_emitters);
         plan.start[i] = (uint32_t)off;
         off += (uint64_t)plan.jobs[i].n_tiles * 64u;
@@ -126,11 +128,16 @@ struct BatchShape {
 
 // The launch shape of a batch: the caller's block and chunk, else the batch
 // autotuner's choice for this job set (which may also raise the stack
-// depth), else 128 lanes and the default chunk.
+// depth), else 128 lanes and the default chunk.  Untuned batches whose trees
+// all have at most 8 levels take the register-rich kernels, as single frames
+// do (they shade more than they fetch nodes): the 64-entry-stack kernel when
+// the grid of the 4-wave one holds every pool entry at once, else the
+// 32-entry one.
 BatchShape batch_shape(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n, BatchPlan& plan, uint32_t block,
                        uint32_t chunk)
 {
     BatchShape s{block, chunk};
+    bool tuned = false;
     if (block == 0 && chunk == 0) {
         const uint64_t key = batch_key(jobs, n);
         for (const BatchTuneEntry& t : ctx->tuned_batch) {
@@ -138,11 +145,17 @@ BatchShape batch_shape(vkr_ctx* ctx, const vkr_scene_job* jobs, uint32_t n, Batc
             s.block = t.block;
             s.chunk = t.chunk;
             plan.stack = std::max(plan.stack, t.stack);
+            tuned = true;
             break;
         }
     }
     if (!s.block) s.block = 128u;
     if (!s.chunk) s.chunk = kDefaultChunk;
+    if (!tuned && plan.stack == 24u && plan.tree_depth <= 8u) {
+        const PathVariant v4{s.block, 32u, false, true, plan.pool};
+        const uint64_t resident4 = (uint64_t)persistent_grid(v4, ctx->device) * s.block;
+        plan.stack = plan.total <= resident4 ? 64u : 32u;
+    }
     return s;
 }
 
